@@ -1,0 +1,246 @@
+#!/usr/bin/env python3
+"""Benchmark of the FP64 complex FFT hot path on one MI355X.
+
+Default workload (the metric's configuration): the batched power-of-two sweep -- complex
+FP64 forward, out of place, N = 2^k for k = 1..18, batch 2^24/N per size (256 MB in and
+256 MB out per size), uniform[-0.5, 0.5) re/im, inputs resident in HBM. One step is one
+pass over all 18 sizes (18 batched transforms). value = algorithmic bytes (32 N per
+transform: 16N read + 16N written) of the whole step divided by the step time, GB/s.
+
+    python bench.py [--steps K] [--warmup W] [--config sweep|single|mixed|ragged|autograd]
+
+Prints one JSON line. `roofline` is measured live with HIP events on the stream the
+kernels run on; `cpu_baseline` times the oracle (the C restatement of the reference's
+refTransform, one thread) on a bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "FP64 c2c FFT GB/s & % HBM roofline; ragged-batch & fwd+bwd vs plain GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def load_package():
+    if "lean_fftw_amd" in sys.modules:
+        return sys.modules["lean_fftw_amd"]
+    pkg = os.path.join(ROOT, "lean-fftw_amd")
+    spec = importlib.util.spec_from_file_location(
+        "lean_fftw_amd", os.path.join(pkg, "__init__.py"), submodule_search_locations=[pkg]
+    )
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["lean_fftw_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def uniform_complex(torch, shape, seed):
+    g = torch.Generator(device="cuda:0").manual_seed(seed)
+    re = torch.rand(shape, dtype=torch.float64, device="cuda:0", generator=g) - 0.5
+    im = torch.rand(shape, dtype=torch.float64, device="cuda:0", generator=g) - 0.5
+    return torch.complex(re, im)
+
+
+def time_steps(torch, step, steps, warmup):
+    """W untimed steps, then K timed steps bracketed by synchronisation; HIP events on the
+    current stream. Returns seconds per step."""
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / steps
+
+
+# ------------------------------------------------------------------------------ CPU baseline
+
+
+def cpu_baseline_sweep(sizes, plans_radices, seconds):
+    """The oracle (C restatement of refTransform, one thread) on a bounded sample of each
+    size of the sweep: `pts` points per size, min of 3 runs, scaled to batch 2^24/N."""
+    sys.path.insert(0, ROOT)
+    import numpy as np
+
+    import oracle
+
+    oracle.build()
+    pts = 1 << 20
+    # calibrate the sample so that the whole leg takes about `seconds`
+    total_bytes = 0.0
+    total_time = 0.0
+    t_start = time.perf_counter()
+    for N in sizes:
+        rp = oracle.RefPlan(True, N, plans_radices[N])
+        hm = max(1, pts // N)
+        x = oracle.lcg_input(N, seed=oracle.LCG_SEED ^ N, howmany=hm)
+        x = x.reshape(hm, N)
+        out = np.empty_like(x)
+        best = float("inf")
+        for _ in range(3):
+            t0 = time.perf_counter()
+            rp.execute(x, out)
+            best = min(best, time.perf_counter() - t0)
+        full = best * ((1 << 24) // N) / hm
+        total_time += full
+        total_bytes += 32.0 * (1 << 24)
+        if time.perf_counter() - t_start > 4 * seconds:
+            break
+    return {
+        "value": round(total_bytes / total_time / 1e9, 3),
+        "unit": "GB/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"oracle refTransform (C, gcc -O2, one thread) with the GPU plans' radices, "
+                  f"{pts} points per size for each of the {len(sizes)} sweep sizes, min of 3, "
+                  f"time scaled to batch 2^24/N; host {platform.processor() or platform.machine()}, "
+                  f"{os.cpu_count()} logical CPUs",
+    }
+
+
+# ------------------------------------------------------------------------------ workloads
+
+
+def bench_sweep(args, torch, lf):
+    sizes = [1 << k for k in range(1, 19)]
+    total = 1 << 24
+    x = uniform_complex(torch, (total,), 20261203)
+    y = torch.empty_like(x)
+    plans = {N: lf.Plan(N, lf.FORWARD, total // N) for N in sizes}
+    views = {N: (x.view(total // N, N), y.view(total // N, N)) for N in sizes}
+
+    def step():
+        for N in sizes:
+            xi, yi = views[N]
+            plans[N].call(xi, yi)
+
+    t = time_steps(torch, step, args.steps, args.warmup)
+    # per-size timing (events around each batched transform), for the roofline breakdown
+    per = []
+    for N in sizes:
+        xi, yi = views[N]
+        for _ in range(max(1, args.warmup)):
+            plans[N].call(xi, yi)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = max(3, args.steps)
+        e0.record()
+        for _ in range(reps):
+            plans[N].call(xi, yi)
+        e1.record()
+        torch.cuda.synchronize()
+        ts = e0.elapsed_time(e1) / 1e3 / reps
+        gbs = 32.0 * total / ts / 1e9
+        per.append({"N": N, "radices": plans[N].radices, "stages": plans[N].stages,
+                    "us": round(ts * 1e6, 2), "GB/s": round(gbs, 1),
+                    "frac": round(gbs / HBM_PEAK_GBS, 4)})
+    bytes_step = 32.0 * total * len(sizes)
+    value = bytes_step / t / 1e9
+    t_sum = sum(p["us"] for p in per) / 1e6
+    achieved = bytes_step / t_sum / 1e9
+    res = {
+        "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "batched power-of-two sweep: complex FP64 forward, out of place, "
+                               "N=2^1..2^18, batch 2^24/N per size (256 MB in + 256 MB out), "
+                               "uniform[-0.5,0.5) re/im; one step = all 18 sizes",
+                   "sizes": len(sizes), "elements_per_size": total},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "stage_kernel<true> (all launches of the sweep; algorithmic "
+                               "32N bytes per transform over summed per-size event time)"},
+        "per_size": per,
+    }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline_sweep(sizes, {N: plans[N].radices for N in sizes},
+                                                 args.cpu_seconds)
+    return res
+
+
+def bench_single(args, torch, lf):
+    """Config 1: one N=1024 transform, out of place (latency-bound on a GPU)."""
+    N = 1024
+    x = uniform_complex(torch, (N,), 1)
+    y = torch.empty_like(x)
+    p = lf.Plan(N, lf.FORWARD, 1)
+    reps = 200
+
+    def step():
+        for _ in range(reps):
+            p.call(x, y)
+
+    t = time_steps(torch, step, args.steps, args.warmup) / reps
+    return {"metric": METRIC, "value": round(32.0 * N / t / 1e9, 4), "unit": "GB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t * 1e3, 6),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "single transform N=1024, forward, out of place",
+                       "us_per_transform": round(t * 1e6, 3), "radices": p.radices}}
+
+
+def bench_mixed(args, torch, lf):
+    """Config 3: batched 13-smooth sizes, each moving >= 256 MB."""
+    sizes = [6, 60, 360, 1000, 2100, 10368, 75600, 165375]
+    per, tot_b, tot_t = [], 0.0, 0.0
+    for N in sizes:
+        hm = -(-(1 << 24) // N)
+        x = uniform_complex(torch, (hm, N), N)
+        y = torch.empty_like(x)
+        p = lf.Plan(N, lf.FORWARD, hm)
+        t = time_steps(torch, lambda: p.call(x, y), args.steps, args.warmup)
+        b = 32.0 * N * hm
+        per.append({"N": N, "batch": hm, "radices": p.radices, "stages": p.stages,
+                    "us": round(t * 1e6, 2), "GB/s": round(b / t / 1e9, 1),
+                    "frac": round(b / t / 1e9 / HBM_PEAK_GBS, 4)})
+        tot_b += b
+        tot_t += t
+        del x, y
+    v = tot_b / tot_t / 1e9
+    return {"metric": METRIC, "value": round(v, 2), "unit": "GB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(tot_t * 1e3, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "batched 13-smooth sizes {6,60,360,1000,2100,10368,75600,165375}, "
+                                   "batch ceil(2^24/N)"},
+            "roofline": {"bound": "hbm", "achieved": round(v, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(v / HBM_PEAK_GBS, 4), "traffic": None},
+            "per_size": per}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="sweep", choices=["sweep", "single", "mixed"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (a batch shards "
+              "with no communication; see the design notes). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    lf = load_package()
+    fn = {"sweep": bench_sweep, "single": bench_single, "mixed": bench_mixed}[args.config]
+    res = fn(args, torch, lf)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

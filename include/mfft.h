@@ -1,0 +1,138 @@
+/* mfft: MI355X (gfx950) double-precision complex FFT -- the C ABI.
+ *
+ * This is the drop-in boundary for the reference's fast complex path. Every entry point
+ * below names the reference interface it replaces (paths relative to the reference
+ * repository). The reference binds its engine from C through these:
+ *
+ *   lean_object *lfa_plan_measure(size_t N, uint8_t fwd, size_t in, size_t out);
+ *        LeanFFTW/Asm/Planner.lean:291-302, declared bench/backends/bench-leanfftw.c:225
+ *   void lfa_call(const void *record, const double *in, double *out);
+ *        LeanFFTW/Asm/Kernels.lean:1203-1204, declared bench/backends/bench-leanfftw.c:226
+ *   lean_object *lfa_set_threads(size_t n);          LeanFFTW/Asm/Planner.lean:305-306
+ *   lfftw_generic_plan(FloatArray spec) / lfftw_generic_exec(plan, in, out)
+ *        LeanFFTW/FFI.lean:50-58 (spec layout :13-21)
+ *
+ * Conventions (docs/CONTRACT.md:9-17 of the reference): forward is sign -1,
+ * Y[k] = sum_j x[j] exp(-2 pi i j k / N); backward is sign +1; nothing is normalised.
+ * Data is interleaved complex binary64 (re, im). Sizes: 2 <= N <= 2^24 with prime factors
+ * <= 13, as the reference's fast engine (bench-leanfftw.c:253-268).
+ *
+ * Differences from the reference, by design:
+ *   - Device pointers: the execute calls take device (HBM) pointers and a HIP stream and
+ *     enqueue asynchronously. mfft_execute_host is the host-pointer convenience.
+ *   - Planning never reads or writes user arrays (lfa_plan_measure overwrites them).
+ *   - Errors are return codes (lfa_call has none).
+ */
+#ifndef MFFT_H
+#define MFFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFFT_FORWARD (-1)
+#define MFFT_BACKWARD (+1)
+
+/* planner flags */
+#define MFFT_ESTIMATE 0u  /* heuristic radix/tile choice, no timing */
+#define MFFT_MEASURE 1u   /* autotune by timing candidate schedules on scratch buffers */
+
+/* return codes */
+#define MFFT_SUCCESS 0
+#define MFFT_ERR_INVALID 1     /* bad argument (null plan, misaligned or null pointer) */
+#define MFFT_ERR_UNSUPPORTED 2 /* size/layout outside the engine's domain */
+#define MFFT_ERR_HIP 3         /* a HIP runtime call failed */
+#define MFFT_ERR_ALLOC 4       /* device or host allocation failed */
+
+typedef struct mfft_plan mfft_plan;
+typedef struct mfft_ragged mfft_ragged;
+
+/* Plan `howmany` transforms of size n with arbitrary element strides and batch
+ * distances (in complex elements), on the current HIP device. Returns NULL if the
+ * problem is unsupported (replaces the empty plan array of lfa_plan_measure,
+ * Planner.lean:297-302, and the `none` of lfftw_generic_plan, FFI.lean:50-52). */
+mfft_plan *mfft_plan_many(int64_t n, int sign, int64_t howmany, int64_t istride,
+                          int64_t idist, int64_t ostride, int64_t odist, unsigned flags);
+
+/* Contiguous batch: istride = ostride = 1, idist = odist = n.
+ * Replaces lfa_plan_measure(N, fwd, in, out) (with howmany = 1). */
+mfft_plan *mfft_plan_dft_1d(int64_t n, int sign, int64_t howmany, unsigned flags);
+
+/* Plan from the reference's generic problem spec (FFI.lean:13-21):
+ * [kind, sign, rank, vrank, inRe, inIm, outRe, outIm, (n,is,os) x rank, (n,is,os) x vrank]
+ * in units of doubles. Supported: kind 0 (complex DFT), rank 1, vrank 0 or 1, interleaved
+ * (inIm = inRe + 1, outIm = outRe + 1, even strides). Offsets are applied by the caller
+ * through the pointers given to mfft_execute_spec. Returns NULL otherwise.
+ * Replaces lfftw_generic_plan (FFI.lean:50-52). */
+mfft_plan *mfft_plan_spec(const double *spec, size_t len, unsigned flags);
+
+/* Execute on device pointers (in == out allowed), enqueued on `stream`
+ * (a hipStream_t; NULL = the default stream). Replaces lfa_call (Kernels.lean:1203-1204). */
+int mfft_execute(const mfft_plan *p, const void *in, void *out, void *stream);
+
+/* Execute with host pointers: copy in, transform, copy out, synchronise. For the
+ * libbench2-shaped backend (bench-leanfftw.c:281-291) and host-array callers. */
+int mfft_execute_host(const mfft_plan *p, const double *in, double *out);
+
+/* Execute a plan made by mfft_plan_spec: `in`/`out` point at the doubles the spec's
+ * offsets are relative to (device pointers). Replaces lfftw_generic_exec (FFI.lean:56-58). */
+int mfft_execute_spec(const mfft_plan *p, const double *in, double *out, void *stream);
+
+void mfft_destroy_plan(mfft_plan *p);
+
+/* Plan description: the radix sequence R_0..R_{s-1} of the Stockham schedule (the
+ * reference's `rs`, Stockham.lean:112) and the number of passes fused into each kernel
+ * stage. Return the count written (or needed, if max is too small). */
+int mfft_plan_radices(const mfft_plan *p, int *radices, int max);
+int mfft_plan_stages(const mfft_plan *p, int *passes_per_stage, int max);
+
+/* Force a radix sequence (tests and tuning): like mfft_plan_many with ESTIMATE, but the
+ * Stockham schedule is `radices` (product n, each radix in {2..16} with prime factors
+ * <= 13) and, if stage_split > 0, the first kernel stage runs passes [0, stage_split). */
+mfft_plan *mfft_plan_with_radices(int64_t n, int sign, int64_t howmany, const int *radices,
+                                  int nradix, int stage_split);
+
+/* Ragged batch: `count` transforms of sizes sizes[i], packed back to back (transform i
+ * starts at element sum_{j<i} sizes[j]), one direction, executed as one persistent
+ * launch per kernel stage from a size-sorted work queue. No counterpart in the reference
+ * (it has no batched fast path, bench-leanfftw.c:257). */
+mfft_ragged *mfft_plan_ragged(int64_t count, const int64_t *sizes, int sign, unsigned flags);
+int mfft_execute_ragged(const mfft_ragged *p, const void *in, void *out, void *stream);
+/* Synchronise `stream` and report whether the last ragged launch completed every tile
+ * (MFFT_SUCCESS) or hit the bound of a stage-B wait (MFFT_ERR_HIP). */
+int mfft_ragged_status(const mfft_ragged *p, void *stream);
+void mfft_destroy_ragged(mfft_ragged *p);
+
+/* Fused weighted transform for the differentiable path: out = diag(post) F (diag(pre) in),
+ * with optional real weights pre[n] and post[n] (NULL = 1) shared by every transform of
+ * the batch. The adjoint of a forward plan is the backward plan with pre and post
+ * exchanged. No counterpart in the reference. */
+int mfft_execute_weighted(const mfft_plan *p, const void *in, void *out, const double *pre,
+                          const double *post, void *stream);
+
+/* The reference's twiddle routine (LeanFFTW/Core/Trig.lean:126, :90-92), exposed for
+ * tests: (cos, sin)(2 pi k / n) and tan(2 pi num / den) as binary64. */
+void mfft_cos_sin_2pi(uint64_t k, uint64_t n, double *c, double *s);
+double mfft_tan_2pi(uint64_t num, uint64_t den);
+
+/* The reference's pass table (LeanFFTW/Core/FastPlan.lean:27-37): for r in [1, R), ns
+ * real parts then ns imaginary parts (2 (R-1) ns doubles). */
+int mfft_pass_twiddles(int fwd, int64_t N, int R, int64_t ns, double *out);
+
+/* Accepted for source compatibility with lfa_set_threads (Planner.lean:305-306); the GPU
+ * grid replaces the CPU worker pool, so this only records the value. */
+int mfft_set_threads(size_t n);
+
+/* Autotuner cache ("wisdom"), JSON lines keyed by (device, n, sign, batch class). */
+int mfft_set_wisdom_path(const char *path);
+
+const char *mfft_error_string(int code);
+const char *mfft_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,328 @@
+"""mfft: MI355X (gfx950) FP64 complex FFT -- Python binding of the C ABI (include/mfft.h).
+
+The shipped compute path is libmfft.so (HIP kernels for gfx950, built in-tree by
+build.py). This module loads it with ctypes and wraps it for torch tensors; torch is
+used only for device memory and streams. There is no CPU fallback: without the built
+library every entry point raises.
+
+Mirrors the reference's plan/execute surface:
+  plan_measure(N, fwd)           <- lfa_plan_measure (LeanFFTW/Asm/Planner.lean:291-302)
+  Plan.call(inp, out)            <- lfa_call (LeanFFTW/Asm/Kernels.lean:1203-1204)
+  Plan.exec(inp) / exec_into     <- Native.exec / execInto (LeanFFTW/Native/Exec.lean:90-95)
+  plan_problem(spec) / exec_spec <- lfftw_generic_plan / exec (LeanFFTW/FFI.lean:50-58)
+  set_threads(n)                 <- lfa_set_threads (LeanFFTW/Asm/Planner.lean:305-306)
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libmfft.so")
+
+FORWARD = -1
+BACKWARD = 1
+ESTIMATE = 0
+MEASURE = 1
+
+_lock = threading.Lock()
+_lib = None
+
+
+class MfftError(RuntimeError):
+    pass
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def lib():
+    """Load libmfft.so (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise MfftError(
+                    f"{LIB_PATH} is missing: build it with `python lean-fftw_amd/build.py` "
+                    "(there is no CPU fallback)"
+                )
+            L = ctypes.CDLL(LIB_PATH)
+            vp, i64, dp = ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)
+            sig = {
+                "mfft_plan_many": (vp, [i64, ctypes.c_int, i64, i64, i64, i64, i64, ctypes.c_uint]),
+                "mfft_plan_dft_1d": (vp, [i64, ctypes.c_int, i64, ctypes.c_uint]),
+                "mfft_plan_spec": (vp, [dp, ctypes.c_size_t, ctypes.c_uint]),
+                "mfft_plan_with_radices": (vp, [i64, ctypes.c_int, i64, ctypes.POINTER(ctypes.c_int),
+                                                ctypes.c_int, ctypes.c_int]),
+                "mfft_execute": (ctypes.c_int, [vp, vp, vp, vp]),
+                "mfft_execute_host": (ctypes.c_int, [vp, dp, dp]),
+                "mfft_execute_spec": (ctypes.c_int, [vp, vp, vp, vp]),
+                "mfft_execute_weighted": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
+                "mfft_destroy_plan": (None, [vp]),
+                "mfft_plan_radices": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
+                "mfft_plan_stages": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
+                "mfft_plan_ragged": (vp, [i64, ctypes.POINTER(ctypes.c_int64), ctypes.c_int, ctypes.c_uint]),
+                "mfft_execute_ragged": (ctypes.c_int, [vp, vp, vp, vp]),
+                "mfft_ragged_status": (ctypes.c_int, [vp, vp]),
+                "mfft_destroy_ragged": (None, [vp]),
+                "mfft_cos_sin_2pi": (None, [ctypes.c_uint64, ctypes.c_uint64, dp, dp]),
+                "mfft_tan_2pi": (ctypes.c_double, [ctypes.c_uint64, ctypes.c_uint64]),
+                "mfft_pass_twiddles": (ctypes.c_int, [ctypes.c_int, i64, ctypes.c_int, i64, dp]),
+                "mfft_set_threads": (ctypes.c_int, [ctypes.c_size_t]),
+                "mfft_set_wisdom_path": (ctypes.c_int, [ctypes.c_char_p]),
+                "mfft_error_string": (ctypes.c_char_p, [ctypes.c_int]),
+                "mfft_version": (ctypes.c_char_p, []),
+            }
+            for name, (res, args) in sig.items():
+                f = getattr(L, name)
+                f.restype = res
+                f.argtypes = args
+            _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise MfftError(f"{what}: {lib().mfft_error_string(rc).decode()} ({rc})")
+
+
+def _stream_ptr(stream):
+    import torch
+
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+# ------------------------------------------------------------------------------ twiddles
+
+
+def cos_sin_2pi(k: int, n: int) -> tuple[float, float]:
+    c, s = ctypes.c_double(), ctypes.c_double()
+    lib().mfft_cos_sin_2pi(k, n, ctypes.byref(c), ctypes.byref(s))
+    return c.value, s.value
+
+
+def tan_2pi(num: int, den: int) -> float:
+    return lib().mfft_tan_2pi(num, den)
+
+
+def pass_twiddles(fwd: bool, N: int, R: int, ns: int):
+    import numpy as np
+
+    out = np.empty(2 * (R - 1) * ns, dtype=np.float64)
+    _check(lib().mfft_pass_twiddles(1 if fwd else 0, N, R, ns,
+                                    out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+           "mfft_pass_twiddles")
+    return out
+
+
+def set_threads(n: int) -> None:
+    _check(lib().mfft_set_threads(n), "mfft_set_threads")
+
+
+# ------------------------------------------------------------------------------ plans
+
+
+class Plan:
+    """A batched 1-D complex FP64 transform on the current device.
+
+    Plan(n, sign=FORWARD, howmany=1, radices=None, split=0, istride=1, idist=None,
+    ostride=1, odist=None, flags=ESTIMATE). `radices` forces the Stockham radix sequence
+    (the reference's `rs`); `split` the number of passes of the first kernel stage.
+    """
+
+    def __init__(self, n, sign=FORWARD, howmany=1, radices=None, split=0, istride=1,
+                 idist=None, ostride=1, odist=None, flags=ESTIMATE, _handle=None):
+        L = lib()
+        self.n, self.sign, self.howmany = int(n), int(sign), int(howmany)
+        self.istride, self.ostride = int(istride), int(ostride)
+        self.idist = int(n if idist is None else idist)
+        self.odist = int(n if odist is None else odist)
+        if _handle is not None:
+            h = _handle
+        elif radices is not None:
+            if (self.istride, self.ostride, self.idist, self.odist) != (1, 1, self.n, self.n):
+                raise ValueError("forced radices are for contiguous batches")
+            arr = (ctypes.c_int * len(radices))(*radices)
+            h = L.mfft_plan_with_radices(self.n, self.sign, self.howmany, arr, len(radices), int(split))
+        else:
+            h = L.mfft_plan_many(self.n, self.sign, self.howmany, self.istride, self.idist,
+                                 self.ostride, self.odist, flags)
+        if not h:
+            raise MfftError(f"unsupported problem: n={n} sign={sign} howmany={howmany} radices={radices}")
+        self._h = ctypes.c_void_p(h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.mfft_destroy_plan(h)
+            self._h = None
+
+    @property
+    def radices(self) -> list[int]:
+        buf = (ctypes.c_int * 32)()
+        n = lib().mfft_plan_radices(self._h, buf, 32)
+        return list(buf[:n])
+
+    @property
+    def stages(self) -> list[int]:
+        buf = (ctypes.c_int * 8)()
+        n = lib().mfft_plan_stages(self._h, buf, 8)
+        return list(buf[:n])
+
+    def _check_tensor(self, t, what):
+        import torch
+
+        if not isinstance(t, torch.Tensor) or t.dtype != torch.complex128 or not t.is_cuda:
+            raise TypeError(f"{what} must be a complex128 CUDA tensor")
+        span = (self.n - 1) * (self.istride if what == "input" else self.ostride) + (
+            self.howmany - 1) * (self.idist if what == "input" else self.odist) + 1
+        if t.numel() < span:
+            raise ValueError(f"{what} holds {t.numel()} elements, the plan spans {span}")
+
+    def call(self, inp, out, stream=None):
+        """out <- transform(inp) on device tensors (inp may be out); asynchronous."""
+        self._check_tensor(inp, "input")
+        self._check_tensor(out, "output")
+        _check(lib().mfft_execute(self._h, ctypes.c_void_p(inp.data_ptr()),
+                                  ctypes.c_void_p(out.data_ptr()), _stream_ptr(stream)),
+               "mfft_execute")
+        return out
+
+    def call_weighted(self, inp, out, pre=None, post=None, stream=None):
+        """out <- diag(post) F diag(pre) inp, with real float64 weights of length n."""
+        self._check_tensor(inp, "input")
+        self._check_tensor(out, "output")
+        import torch
+
+        for w in (pre, post):
+            if w is not None and (w.dtype != torch.float64 or not w.is_cuda or w.numel() < self.n):
+                raise TypeError("weights must be float64 CUDA tensors of length n")
+        ptr = lambda w: ctypes.c_void_p(w.data_ptr()) if w is not None else None  # noqa: E731
+        _check(lib().mfft_execute_weighted(self._h, ctypes.c_void_p(inp.data_ptr()),
+                                           ctypes.c_void_p(out.data_ptr()), ptr(pre), ptr(post),
+                                           _stream_ptr(stream)), "mfft_execute_weighted")
+        return out
+
+    def exec(self, inp, stream=None):
+        import torch
+
+        out = torch.empty_like(inp)
+        return self.call(inp, out, stream)
+
+    exec_into = call
+
+    def execute_host(self, inp, out):
+        """Host numpy arrays (complex128): copy in, transform, copy out."""
+        import numpy as np
+
+        assert inp.dtype == np.complex128 and out.dtype == np.complex128
+        dp = ctypes.POINTER(ctypes.c_double)
+        _check(lib().mfft_execute_host(self._h, inp.ctypes.data_as(dp), out.ctypes.data_as(dp)),
+               "mfft_execute_host")
+        return out
+
+
+def plan_measure(N: int, fwd: bool, howmany: int = 1) -> Plan:
+    """lfa_plan_measure without touching user arrays: a plan of N points, one direction."""
+    return Plan(N, FORWARD if fwd else BACKWARD, howmany, flags=MEASURE)
+
+
+class SpecPlan(Plan):
+    """A plan from the reference's generic problem spec (LeanFFTW/FFI.lean:13-21)."""
+
+    def __init__(self, spec, flags=ESTIMATE):
+        import numpy as np
+
+        arr = np.ascontiguousarray(spec, dtype=np.float64)
+        h = lib().mfft_plan_spec(arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), arr.size, flags)
+        if not h:
+            raise MfftError("unsupported problem spec")
+        self._h = ctypes.c_void_p(h)
+        self.n = int(spec[8])
+        self.sign = -1 if spec[1] < 0 else 1
+
+    def call(self, inp_doubles, out_doubles, stream=None):
+        _check(lib().mfft_execute_spec(self._h, ctypes.c_void_p(inp_doubles.data_ptr()),
+                                       ctypes.c_void_p(out_doubles.data_ptr()), _stream_ptr(stream)),
+               "mfft_execute_spec")
+        return out_doubles
+
+
+def plan_problem(spec) -> SpecPlan | None:
+    try:
+        return SpecPlan(spec)
+    except MfftError:
+        return None
+
+
+class RaggedPlan:
+    """Transforms of mixed sizes packed back to back, one persistent launch per call."""
+
+    def __init__(self, sizes, sign=FORWARD, flags=ESTIMATE):
+        import numpy as np
+
+        self.sizes = np.ascontiguousarray(sizes, dtype=np.int64)
+        self.total = int(self.sizes.sum())
+        self.sign = sign
+        h = lib().mfft_plan_ragged(len(self.sizes), self.sizes.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_int64)), sign, flags)
+        if not h:
+            raise MfftError("unsupported ragged batch")
+        self._h = ctypes.c_void_p(h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.mfft_destroy_ragged(h)
+            self._h = None
+
+    def call(self, inp, out, stream=None):
+        import torch
+
+        for t in (inp, out):
+            if t.dtype != torch.complex128 or not t.is_cuda or t.numel() < self.total:
+                raise TypeError("ragged tensors must be complex128 CUDA tensors of the total size")
+        _check(lib().mfft_execute_ragged(self._h, ctypes.c_void_p(inp.data_ptr()),
+                                         ctypes.c_void_p(out.data_ptr()), _stream_ptr(stream)),
+               "mfft_execute_ragged")
+        return out
+
+    def status(self, stream=None):
+        _check(lib().mfft_ragged_status(self._h, _stream_ptr(stream)), "ragged launch")
+
+
+# ------------------------------------------------------------------------------ torch-level API
+
+_plan_cache: dict = {}
+
+
+def _cached_plan(n, sign, howmany):
+    key = (n, sign, howmany)
+    p = _plan_cache.get(key)
+    if p is None:
+        p = Plan(n, sign, howmany)
+        _plan_cache[key] = p
+    return p
+
+
+def fft(x, sign: int = FORWARD):
+    """Unnormalised transform along the last dimension of a complex128 CUDA tensor."""
+    x = x.contiguous()
+    n = x.shape[-1]
+    howmany = x.numel() // n if n else 0
+    out = x.new_empty(x.shape)
+    if howmany:
+        _cached_plan(n, sign, howmany).call(x, out)
+    return out
+
+
+def ifft_unnormalized(x):
+    return fft(x, BACKWARD)
+
+
+from .autograd import FFTFunction, fft_autograd, weighted_power_loss  # noqa: E402,F401

@@ -1,0 +1,540 @@
+// Host side of mfft: plans, twiddle tables, kernel-stage schedules and the C ABI.
+//
+// A plan is the reference's fast plan (a radix sequence rs of a Stockham schedule,
+// LeanFFTW/Gen/Stockham.lean:101-115) plus the way it is cut into GPU kernel stages:
+//
+//   * N <= LDS_ELEMS: one stage; each workgroup runs every pass of G whole transforms
+//     in LDS.
+//   * larger N = P * Q: the first passes (product P) act on the Q decimated sequences
+//     x[c + m Q] (m < P) -- the Stockham property that after passes with product P the
+//     array holds, at c*P + k, the P-point DFT of that sequence -- and the remaining
+//     passes act on the P residue classes {k1 + P i}, each a Q-point Stockham with the
+//     plan's own twiddles (index k1 + P * k_local). Both stages compute exactly the
+//     reference's passes, so the result is bit-identical to refTransform with rs.
+//
+// Twiddles are the reference's pass tables (LeanFFTW/Core/FastPlan.lean:27-37) computed
+// with the exact-integer routine of mfft_trig.cpp, stored on the device per pass as
+// (R-1) rows of ns interleaved (re, im) pairs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/mfft.h"
+#include "bfly_gen.h"
+#include "mfft_internal.h"
+
+using mfft::StageArgs;
+
+struct mfft_plan {
+  int64_t n = 0;
+  int sign = -1;
+  int64_t howmany = 1, istride = 1, idist = 0, ostride = 1, odist = 0;
+  std::vector<int> radices;
+  std::vector<int> stage_passes;
+  std::vector<StageArgs> stages;
+  double2 *d_tw = nullptr;
+  size_t tw_elems = 0;
+  std::vector<double2> h_tw;  // host copy of the pass tables (ragged batches reuse it)
+  double2 *d_scratch = nullptr;  // multi-stage intermediate, chunk_h transforms
+  int64_t chunk_h = 0;
+  int device = 0;
+  int64_t spec_in_off = 0, spec_out_off = 0;  // doubles, for mfft_plan_spec plans
+};
+
+namespace {
+
+std::atomic<size_t> g_threads{1};
+std::mutex g_wisdom_mu;
+std::string g_wisdom_path;
+
+// Bytes of intermediate per chunk of a two-stage plan: small enough that the chunk's
+// intermediate stays in the 256 MB Infinity Cache between the two stages.
+int64_t chunk_bytes() {
+  static const int64_t v = [] {
+    const char *e = std::getenv("MFFT_CHUNK_MB");
+    const int64_t mb = e ? std::atoll(e) : 64;
+    return std::max<int64_t>(1, mb) << 20;
+  }();
+  return v;
+}
+
+bool smooth13(int64_t n) {
+  for (int p : {2, 3, 5, 7, 11, 13})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+bool radix_ok(int r) { return r >= 2 && r <= 16 && smooth13(r); }
+
+// Cost model of a pass: arithmetic of the generated twiddled butterfly per point plus a
+// fixed charge for the LDS round trip of the pass.
+double pass_cost(int r) { return (double)mfft::kBflyOps[r] / r + 6.0; }
+
+// Radix sequence for an LDS stage of length L: the fewest passes (every pass is one
+// LDS round trip), then the least arithmetic; radices in descending order.
+std::vector<int> choose_radices(int64_t L) {
+  // memoised DP over divisors
+  std::vector<std::pair<double, std::vector<int>>> best;
+  std::vector<int64_t> divs;
+  for (int64_t d = 1; d <= L; ++d)
+    if (L % d == 0) divs.push_back(d);
+  best.assign(divs.size(), {1e300, {}});
+  best[0] = {0.0, {}};
+  for (size_t i = 1; i < divs.size(); ++i) {
+    const int64_t d = divs[i];
+    for (int r = 16; r >= 2; --r) {
+      if (!radix_ok(r) || d % r) continue;
+      const size_t j = std::lower_bound(divs.begin(), divs.end(), d / r) - divs.begin();
+      if (best[j].first >= 1e299) continue;
+      // lexicographic (passes, cost): a pass costs 1000 + arithmetic
+      const double c = best[j].first + 1000.0 + pass_cost(r);
+      if (c < best[i].first) {
+        best[i].first = c;
+        best[i].second = best[j].second;
+        best[i].second.push_back(r);
+      }
+    }
+  }
+  std::vector<int> rs = best.back().second;
+  std::sort(rs.rbegin(), rs.rend());
+  return rs;
+}
+
+// Split N into P * Q for a two-stage plan: both within the LDS, as balanced as possible.
+bool choose_split(int64_t N, int64_t &P) {
+  P = 0;
+  double best = 1e300;
+  for (int64_t p = 2; p <= mfft::LDS_ELEMS && p < N; ++p) {
+    if (N % p) continue;
+    const int64_t q = N / p;
+    if (q > mfft::LDS_ELEMS) continue;
+    const double score = std::fabs(std::log((double)p) - std::log((double)q));
+    if (score < best) {
+      best = score;
+      P = p;
+    }
+  }
+  return P != 0;
+}
+
+hipError_t build_twiddles(mfft_plan *p) {
+  const int64_t N = p->n;
+  const bool fwd = p->sign < 0;
+  const auto &rs = p->radices;
+  std::vector<double> tab;
+  if (rs.size() > 1) mfft::cos_sin_2pi_table((uint64_t)N, tab);
+  std::vector<double2> host;
+  int64_t ns = 1;
+  std::vector<int64_t> off(rs.size(), -1);
+  for (size_t t = 0; t < rs.size(); ++t) {
+    const int R = rs[t];
+    if (t >= 1) {
+      off[t] = (int64_t)host.size();
+      const int64_t step = N / (ns * R);
+      for (int r = 1; r < R; ++r)
+        for (int64_t k = 0; k < ns; ++k) {
+          const int64_t idx = (int64_t)r * k * step;
+          const double c = tab[2 * idx], s = tab[2 * idx + 1];
+          host.push_back(make_double2(c, fwd ? (s == 0.0 ? 0.0 : -s) : s));
+        }
+    }
+    ns *= R;
+  }
+  p->tw_elems = host.size();
+  p->h_tw = host;
+  if (!host.empty()) {
+    hipError_t e = hipMalloc(&p->d_tw, host.size() * sizeof(double2));
+    if (e != hipSuccess) return e;
+    e = hipMemcpy(p->d_tw, host.data(), host.size() * sizeof(double2), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+  }
+  // stamp the offsets into the stages
+  size_t t = 0;
+  for (auto &st : p->stages)
+    for (int q = 0; q < st.npass; ++q, ++t) st.twoff[q] = off[t];
+  return hipSuccess;
+}
+
+// Build the stage descriptors of a plan whose radices and split are set.
+bool build_stages(mfft_plan *p, int split) {
+  const int64_t N = p->n;
+  const auto &rs = p->radices;
+  const int s = (int)rs.size();
+  p->stages.clear();
+  p->stage_passes.clear();
+  std::vector<int64_t> ns(s + 1, 1);
+  for (int t = 0; t < s; ++t) ns[t + 1] = ns[t] * rs[t];
+  if (split <= 0 || split >= s) {
+    if (N > mfft::LDS_ELEMS || s > mfft::MAX_PASSES) return false;
+    StageArgs a{};
+    a.L = (int)N;
+    a.npass = s;
+    for (int t = 0; t < s; ++t) {
+      a.R[t] = rs[t];
+      a.nsl[t] = (int)ns[t];
+      a.twns[t] = ns[t];
+    }
+    a.kmul = 1;
+    a.kb_u = 0;
+    a.U = 1;
+    a.in_u = 0;
+    a.in_i = p->istride;
+    a.out_u = 0;
+    a.out_i = p->ostride;
+    a.idist = p->idist;
+    a.odist = p->odist;
+    a.nsub = p->howmany;
+    a.G = (int)std::max<int64_t>(1, std::min<int64_t>(mfft::LDS_ELEMS / N, p->howmany));
+    a.gfast_in = std::llabs(p->idist) < std::llabs(p->istride) ? 1 : 0;
+    a.gfast_out = std::llabs(p->odist) < std::llabs(p->ostride) ? 1 : 0;
+    a.pre_u = 0;
+    a.pre_i = 1;
+    a.post_u = 0;
+    a.post_i = 1;
+    p->stages.push_back(a);
+    p->stage_passes.push_back(s);
+    p->chunk_h = p->howmany;
+    return true;
+  }
+  const int64_t P = ns[split], Q = N / P;
+  if (P > mfft::LDS_ELEMS || Q > mfft::LDS_ELEMS || split > mfft::MAX_PASSES ||
+      s - split > mfft::MAX_PASSES)
+    return false;
+  StageArgs A{}, B{};
+  A.L = (int)P;
+  A.npass = split;
+  for (int t = 0; t < split; ++t) {
+    A.R[t] = rs[t];
+    A.nsl[t] = (int)ns[t];
+    A.twns[t] = ns[t];
+  }
+  A.kmul = 1;
+  A.kb_u = 0;
+  A.U = Q;
+  A.in_u = p->istride;
+  A.in_i = p->istride * Q;
+  A.idist = p->idist;
+  A.out_u = P;  // into the contiguous intermediate
+  A.out_i = 1;
+  A.odist = N;
+  A.G = (int)std::max<int64_t>(1, mfft::LDS_ELEMS / P);
+  A.gfast_in = 1;
+  A.gfast_out = 0;
+  A.pre_u = 1;
+  A.pre_i = Q;
+  A.post_u = 0;
+  A.post_i = 0;
+
+  B.L = (int)Q;
+  B.npass = s - split;
+  for (int t = split; t < s; ++t) {
+    B.R[t - split] = rs[t];
+    B.nsl[t - split] = (int)(ns[t] / P);
+    B.twns[t - split] = ns[t];
+  }
+  B.kmul = P;
+  B.kb_u = 1;
+  B.U = P;
+  B.in_u = 1;
+  B.in_i = P;
+  B.idist = N;
+  B.out_u = p->ostride;
+  B.out_i = p->ostride * P;
+  B.odist = p->odist;
+  B.G = (int)std::max<int64_t>(1, mfft::LDS_ELEMS / Q);
+  B.gfast_in = 1;
+  B.gfast_out = 1;
+  B.pre_u = 0;
+  B.pre_i = 0;
+  B.post_u = 1;
+  B.post_i = P;
+  p->stages.push_back(A);
+  p->stages.push_back(B);
+  p->stage_passes.push_back(split);
+  p->stage_passes.push_back(s - split);
+  p->chunk_h = std::max<int64_t>(1, std::min<int64_t>(p->howmany, chunk_bytes() / (16 * N)));
+  return true;
+}
+
+mfft_plan *finish(mfft_plan *p, int split) {
+  if (!build_stages(p, split) || hipGetDevice(&p->device) != hipSuccess ||
+      build_twiddles(p) != hipSuccess) {
+    mfft_destroy_plan(p);
+    return nullptr;
+  }
+  if (p->stages.size() > 1) {
+    if (hipMalloc(&p->d_scratch, (size_t)p->chunk_h * p->n * sizeof(double2)) != hipSuccess) {
+      mfft_destroy_plan(p);
+      return nullptr;
+    }
+  }
+  return p;
+}
+
+bool valid_problem(int64_t n, int sign, int64_t howmany) {
+  return n >= 2 && n <= (int64_t(1) << 24) && smooth13(n) && (sign == -1 || sign == 1) &&
+         howmany >= 0;
+}
+
+int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
+        const double *post, hipStream_t st) {
+  if (p->howmany == 0) return MFFT_SUCCESS;
+  if (!in || !out) return MFFT_ERR_INVALID;
+  const bool fwd = p->sign < 0;
+  if (p->stages.size() == 1) {
+    StageArgs a = p->stages[0];
+    a.pre = pre;
+    a.post = post;
+    return mfft::launch_stage(fwd, a, in, out, p->d_tw, st) == hipSuccess ? MFFT_SUCCESS
+                                                                         : MFFT_ERR_HIP;
+  }
+  const int64_t N = p->n;
+  for (int64_t h0 = 0; h0 < p->howmany; h0 += p->chunk_h) {
+    const int64_t hc = std::min(p->chunk_h, p->howmany - h0);
+    StageArgs A = p->stages[0], B = p->stages[1];
+    A.nsub = hc * A.U;
+    B.nsub = hc * B.U;
+    A.pre = pre;
+    B.post = post;
+    if (mfft::launch_stage(fwd, A, in + h0 * p->idist, p->d_scratch, p->d_tw, st) != hipSuccess)
+      return MFFT_ERR_HIP;
+    if (mfft::launch_stage(fwd, B, p->d_scratch, out + h0 * p->odist, p->d_tw, st) != hipSuccess)
+      return MFFT_ERR_HIP;
+    (void)N;
+  }
+  return MFFT_SUCCESS;
+}
+
+}  // namespace
+
+int mfft::export_class(const mfft_plan *p, mfft::RClass *c, std::vector<double2> *tw) {
+  if (!p || p->howmany != 1 || p->stages.empty() || p->stages.size() > 2) return 1;
+  const int64_t base = (int64_t)tw->size();
+  *c = mfft::RClass{};
+  c->N = (int)p->n;
+  c->nstage = (int)p->stages.size();
+  for (int s = 0; s < c->nstage; ++s) {
+    const StageArgs &a = p->stages[s];
+    c->L[s] = a.L;
+    c->npass[s] = a.npass;
+    c->G[s] = a.G;
+    for (int q = 0; q < a.npass; ++q) {
+      c->R[s][q] = a.R[q];
+      c->nsl[s][q] = a.nsl[q];
+      c->twoff[s][q] = a.twoff[q] >= 0 ? a.twoff[q] + base : -1;
+      c->twns[s][q] = a.twns[q];
+    }
+  }
+  c->P = c->nstage == 2 ? p->stages[0].L : 0;
+  c->Q = c->nstage == 2 ? p->stages[1].L : 0;
+  tw->insert(tw->end(), p->h_tw.begin(), p->h_tw.end());
+  return 0;
+}
+
+extern "C" {
+
+mfft_plan *mfft_plan_with_radices(int64_t n, int sign, int64_t howmany, const int *radices,
+                                  int nradix, int stage_split) {
+  if (!valid_problem(n, sign, howmany) || !radices || nradix < 1) return nullptr;
+  int64_t prod = 1;
+  for (int i = 0; i < nradix; ++i) {
+    if (!radix_ok(radices[i])) return nullptr;
+    prod *= radices[i];
+  }
+  if (prod != n) return nullptr;
+  auto *p = new mfft_plan;
+  p->n = n;
+  p->sign = sign;
+  p->howmany = howmany;
+  p->istride = p->ostride = 1;
+  p->idist = p->odist = n;
+  p->radices.assign(radices, radices + nradix);
+  return finish(p, stage_split);
+}
+
+mfft_plan *mfft_plan_many(int64_t n, int sign, int64_t howmany, int64_t istride, int64_t idist,
+                          int64_t ostride, int64_t odist, unsigned flags) {
+  (void)flags;
+  if (!valid_problem(n, sign, howmany) || istride == 0 || ostride == 0) return nullptr;
+  auto *p = new mfft_plan;
+  p->n = n;
+  p->sign = sign;
+  p->howmany = howmany;
+  p->istride = istride;
+  p->idist = idist;
+  p->ostride = ostride;
+  p->odist = odist;
+  int split = 0;
+  if (n <= mfft::LDS_ELEMS) {
+    p->radices = choose_radices(n);
+  } else {
+    int64_t P;
+    if (!choose_split(n, P)) {
+      delete p;
+      return nullptr;
+    }
+    std::vector<int> a = choose_radices(P), b = choose_radices(n / P);
+    p->radices = a;
+    p->radices.insert(p->radices.end(), b.begin(), b.end());
+    split = (int)a.size();
+  }
+  return finish(p, split);
+}
+
+mfft_plan *mfft_plan_dft_1d(int64_t n, int sign, int64_t howmany, unsigned flags) {
+  return mfft_plan_many(n, sign, howmany, 1, n, 1, n, flags);
+}
+
+mfft_plan *mfft_plan_spec(const double *spec, size_t len, unsigned flags) {
+  // [kind, sign, rank, vrank, inRe, inIm, outRe, outIm, (n,is,os) x rank, (n,is,os) x vrank]
+  if (!spec || len < 8) return nullptr;
+  auto gi = [&](size_t i) { return (int64_t)spec[i]; };
+  const int64_t kind = gi(0), sign = gi(1), rank = gi(2), vrank = gi(3);
+  if (kind != 0 || rank != 1 || vrank < 0 || vrank > 1) return nullptr;
+  if (len < (size_t)(8 + 3 * (rank + vrank))) return nullptr;
+  const int64_t inRe = gi(4), inIm = gi(5), outRe = gi(6), outIm = gi(7);
+  if (inIm != inRe + 1 || outIm != outRe + 1) return nullptr;  // interleaved only
+  const int64_t n = gi(8), is = gi(9), os = gi(10);
+  int64_t hm = 1, vis = 0, vos = 0;
+  if (vrank == 1) {
+    hm = gi(11);
+    vis = gi(12);
+    vos = gi(13);
+  }
+  if ((is % 2) || (os % 2) || (vis % 2) || (vos % 2)) return nullptr;
+  mfft_plan *p = mfft_plan_many(n, sign < 0 ? -1 : 1, hm, is / 2, vis / 2, os / 2, vos / 2, flags);
+  if (p) {
+    p->spec_in_off = inRe;
+    p->spec_out_off = outRe;
+  }
+  return p;
+}
+
+int mfft_execute(const mfft_plan *p, const void *in, void *out, void *stream) {
+  if (!p) return MFFT_ERR_INVALID;
+  return run(p, (const double2 *)in, (double2 *)out, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int mfft_execute_weighted(const mfft_plan *p, const void *in, void *out, const double *pre,
+                          const double *post, void *stream) {
+  if (!p) return MFFT_ERR_INVALID;
+  if ((pre || post) && (p->istride != 1 || p->ostride != 1)) return MFFT_ERR_UNSUPPORTED;
+  return run(p, (const double2 *)in, (double2 *)out, pre, post, (hipStream_t)stream);
+}
+
+int mfft_execute_spec(const mfft_plan *p, const double *in, double *out, void *stream) {
+  if (!p || !in || !out) return MFFT_ERR_INVALID;
+  return run(p, (const double2 *)(in + p->spec_in_off), (double2 *)(out + p->spec_out_off),
+             nullptr, nullptr, (hipStream_t)stream);
+}
+
+int mfft_execute_host(const mfft_plan *p, const double *in, double *out) {
+  if (!p || !in || !out) return MFFT_ERR_INVALID;
+  // span of elements touched on each side
+  auto span = [&](int64_t stride, int64_t dist) {
+    const int64_t a = std::llabs(stride) * (p->n - 1) + std::llabs(dist) * (p->howmany - 1) + 1;
+    return a;
+  };
+  if (p->istride < 0 || p->ostride < 0 || p->idist < 0 || p->odist < 0)
+    return MFFT_ERR_UNSUPPORTED;
+  const int64_t ni = span(p->istride, p->idist), no = span(p->ostride, p->odist);
+  double2 *di = nullptr, *dout = nullptr;
+  if (hipMalloc(&di, ni * sizeof(double2)) != hipSuccess) return MFFT_ERR_ALLOC;
+  if (hipMalloc(&dout, no * sizeof(double2)) != hipSuccess) {
+    (void)hipFree(di);
+    return MFFT_ERR_ALLOC;
+  }
+  int rc = MFFT_SUCCESS;
+  if (hipMemcpy(di, in, ni * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dout, out, no * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess)
+    rc = MFFT_ERR_HIP;
+  if (rc == MFFT_SUCCESS) rc = run(p, di, dout, nullptr, nullptr, nullptr);
+  if (rc == MFFT_SUCCESS && (hipDeviceSynchronize() != hipSuccess ||
+                             hipMemcpy(out, dout, no * sizeof(double2), hipMemcpyDeviceToHost) !=
+                                 hipSuccess))
+    rc = MFFT_ERR_HIP;
+  (void)hipFree(di);
+  (void)hipFree(dout);
+  return rc;
+}
+
+void mfft_destroy_plan(mfft_plan *p) {
+  if (!p) return;
+  if (p->d_tw) (void)hipFree(p->d_tw);
+  if (p->d_scratch) (void)hipFree(p->d_scratch);
+  delete p;
+}
+
+int mfft_plan_radices(const mfft_plan *p, int *radices, int max) {
+  if (!p) return -1;
+  const int n = (int)p->radices.size();
+  for (int i = 0; i < n && i < max; ++i) radices[i] = p->radices[i];
+  return n;
+}
+
+int mfft_plan_stages(const mfft_plan *p, int *passes, int max) {
+  if (!p) return -1;
+  const int n = (int)p->stage_passes.size();
+  for (int i = 0; i < n && i < max; ++i) passes[i] = p->stage_passes[i];
+  return n;
+}
+
+void mfft_cos_sin_2pi(uint64_t k, uint64_t n, double *c, double *s) {
+  double cc, ss;
+  mfft::cos_sin_2pi(k, n, cc, ss);
+  if (c) *c = cc;
+  if (s) *s = ss;
+}
+
+double mfft_tan_2pi(uint64_t num, uint64_t den) { return mfft::tan_2pi(num, den); }
+
+int mfft_pass_twiddles(int fwd, int64_t N, int R, int64_t ns, double *out) {
+  if (!out || N < 1 || R < 1 || ns < 1 || N % (ns * R)) return MFFT_ERR_INVALID;
+  std::vector<double> tab;
+  mfft::cos_sin_2pi_table((uint64_t)N, tab);
+  const int64_t step = N / (ns * R);
+  size_t o = 0;
+  for (int r = 1; r < R; ++r) {
+    for (int64_t k = 0; k < ns; ++k) out[o++] = tab[2 * (r * k * step)];
+    for (int64_t k = 0; k < ns; ++k) {
+      const double s = tab[2 * (r * k * step) + 1];
+      out[o++] = fwd ? (s == 0.0 ? 0.0 : -s) : s;
+    }
+  }
+  return MFFT_SUCCESS;
+}
+
+int mfft_set_threads(size_t n) {
+  g_threads = n;
+  return MFFT_SUCCESS;
+}
+
+int mfft_set_wisdom_path(const char *path) {
+  std::lock_guard<std::mutex> lk(g_wisdom_mu);
+  g_wisdom_path = path ? path : "";
+  return MFFT_SUCCESS;
+}
+
+const char *mfft_error_string(int code) {
+  switch (code) {
+    case MFFT_SUCCESS: return "success";
+    case MFFT_ERR_INVALID: return "invalid argument";
+    case MFFT_ERR_UNSUPPORTED: return "unsupported problem";
+    case MFFT_ERR_HIP: return "HIP runtime error";
+    case MFFT_ERR_ALLOC: return "allocation failed";
+    default: return "unknown error";
+  }
+}
+
+const char *mfft_version(void) { return "mfft 0.1 (gfx950)"; }
+
+}  // extern "C"

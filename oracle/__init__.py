@@ -225,3 +225,28 @@ def lcg_input(N: int, seed: int = LCG_SEED, howmany: int = 1) -> np.ndarray:
     out = (vals[:total] >> np.uint64(11)).astype(np.float64) / 9007199254740992.0 - 0.5
     c = out.view(np.complex128)
     return c if howmany == 1 else c.reshape(howmany, N)
+
+
+class RefPlan:
+    """refTransform with precomputed tables: execute() runs only the compiled pass loop
+    (for timing the oracle as the CPU baseline; tables are planning, as in the reference)."""
+
+    def __init__(self, fwd: bool, N: int, rs):
+        self.fwd, self.N, self.rs = bool(fwd), int(N), [int(r) for r in rs]
+        if math.prod(self.rs) != self.N:
+            raise ValueError("plan does not multiply to N")
+        tw = plan_twiddles(self.fwd, self.N, self.rs)
+        self.tw = tw if tw.size else np.zeros(1)
+        self.rsa = (ctypes.c_int * len(self.rs))(*self.rs)
+        self.work = np.empty(4 * self.N)
+
+    def execute(self, x: np.ndarray, out: np.ndarray) -> None:
+        """x, out: complex128 arrays of shape (howmany, N), C-contiguous."""
+        howmany = x.size // self.N
+        rc = lib().oracle_ref_transform(
+            1 if self.fwd else 0, self.N, len(self.rs), self.rsa, _dp(self.tw), howmany,
+            _dp(x.reshape(-1).view(np.float64)), _dp(out.reshape(-1).view(np.float64)),
+            _dp(self.work),
+        )
+        if rc != 0:
+            raise ValueError("oracle cannot run this plan")
