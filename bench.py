@@ -78,7 +78,7 @@ def cpu_baseline_sweep(sizes, plans_radices, seconds):
     import oracle
 
     oracle.build()
-    pts = 1 << 20
+    pts = 1 << 22
     # calibrate the sample so that the whole leg takes about `seconds`
     total_bytes = 0.0
     total_time = 0.0
@@ -221,12 +221,102 @@ def bench_mixed(args, torch, lf):
             "per_size": per}
 
 
+def bench_ragged(args, torch, lf):
+    """Config 4: 10,000 transforms of log-uniform 13-smooth sizes in [8, 65536], packed back
+    to back, one persistent launch, captured in a HIP graph. Compared with per-size plain
+    launches: the same transforms regrouped by size (contiguous per size) and run as one
+    batched uniform launch per distinct size."""
+    sizes = lf.workloads.ragged_sizes()
+    total = sum(sizes)
+    x = uniform_complex(torch, (total,), 44)
+    y = torch.empty_like(x)
+    t0 = time.perf_counter()
+    rp = lf.RaggedPlan(sizes, lf.FORWARD)
+    plan_s = time.perf_counter() - t0
+    rp.call(x, y)
+    rp.status()
+    graph = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        rp.call(x, y)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph, stream=s):
+            rp.call(x, y)
+    torch.cuda.current_stream().wait_stream(s)
+    t_rag = time_steps(torch, graph.replay, args.steps, args.warmup)
+    rp.status()
+    b = 32.0 * total
+    # per-size plain launches on the regrouped layout
+    from collections import Counter
+
+    cnt = Counter(sizes)
+    plans = {n: lf.Plan(n, lf.FORWARD, c) for n, c in cnt.items()}
+    offs, off = {}, 0
+    for n, c in sorted(cnt.items()):
+        offs[n] = off
+        off += n * c
+
+    def per_size():
+        for n, c in cnt.items():
+            o = offs[n]
+            plans[n].call(x[o:o + n * c], y[o:o + n * c])
+
+    t_ps = time_steps(torch, per_size, args.steps, args.warmup)
+    return {"metric": METRIC, "value": round(b / t_rag / 1e9, 2), "unit": "GB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_rag * 1e3, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "ragged batch: 10,000 complex FP64 forward transforms, sizes "
+                                   "log-uniform over the 13-smooth integers in [8, 65536] (PCG64 "
+                                   "seed 20261203), packed back to back, one persistent launch in "
+                                   "a HIP graph", "sum_N": total, "distinct_sizes": len(cnt),
+                       "plan_seconds": round(plan_s, 2)},
+            "per_size_plain": {"GB/s": round(b / t_ps / 1e9, 2), "ms": round(t_ps * 1e3, 4),
+                               "launches": len(cnt)},
+            "ragged_vs_per_size": round(t_ps / t_rag, 4),
+            "roofline": {"bound": "hbm", "achieved": round(b / t_rag / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(b / t_rag / 1e9 / HBM_PEAK_GBS, 4),
+                         "traffic": None}}
+
+
+def bench_autograd(args, torch, lf):
+    """Config 5: loss = sum(|w * FFT(x)|^2), x (512, 4096) complex128; forward alone vs
+    forward + backward (the hand-written adjoint)."""
+    B, N = 512, 4096
+    x = uniform_complex(torch, (B, N), 55).requires_grad_(True)
+    g = torch.Generator(device="cuda:0").manual_seed(56)
+    w = torch.rand(N, dtype=torch.float64, device="cuda:0", generator=g)
+
+    def fwd():
+        with torch.no_grad():
+            lf.weighted_power_loss(x, w)
+
+    def fwd_bwd():
+        x.grad = None
+        loss = lf.weighted_power_loss(x, w)
+        loss.backward()
+
+    t_f = time_steps(torch, fwd, args.steps * 4, args.warmup)
+    t_fb = time_steps(torch, fwd_bwd, args.steps * 4, args.warmup)
+    b = 32.0 * B * N
+    return {"metric": METRIC, "value": round(t_fb / t_f, 4), "unit": "x (fwd+bwd time / fwd time)",
+            "n_gpus": 1, "steps": args.steps * 4, "warmup": args.warmup,
+            "ms_per_step": round(t_fb * 1e3, 4), "higher_is_better": False, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "differentiable FFT: loss = sum(|w*FFT(x)|^2), x (512, 4096) "
+                                   "complex128, w ~ U[0,1)", "fwd_ms": round(t_f * 1e3, 4),
+                       "fwd_bwd_ms": round(t_fb * 1e3, 4),
+                       "fwd_fft_GB/s": round(b / t_f / 1e9, 1)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="sweep", choices=["sweep", "single", "mixed"])
+    ap.add_argument("--config", default="sweep",
+                    choices=["sweep", "single", "mixed", "ragged", "autograd"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
@@ -237,7 +327,8 @@ def main():
     import torch
 
     lf = load_package()
-    fn = {"sweep": bench_sweep, "single": bench_single, "mixed": bench_mixed}[args.config]
+    fn = {"sweep": bench_sweep, "single": bench_single, "mixed": bench_mixed,
+          "ragged": bench_ragged, "autograd": bench_autograd}[args.config]
     res = fn(args, torch, lf)
     print(json.dumps(res), flush=True)
 

@@ -325,4 +325,5 @@ def ifft_unnormalized(x):
     return fft(x, BACKWARD)
 
 
+from . import workloads  # noqa: E402,F401
 from .autograd import FFTFunction, fft_autograd, weighted_power_loss  # noqa: E402,F401

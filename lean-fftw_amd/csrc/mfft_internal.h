@@ -20,43 +20,34 @@ constexpr int MAX_PASSES = 12;      // passes per kernel stage
 constexpr int WG_THREADS = 256;     // threads per workgroup of the stage kernels
 constexpr int LDS_ELEMS = 2048;     // complex doubles per LDS buffer (two buffers: 64 KiB)
 
-// One kernel stage: a group of consecutive Stockham passes run on independent
-// sub-problems ("sub-transforms") held in LDS.
+// One Stockham pass inside a kernel stage. Local index k of butterfly j is j mod nsl (nsl:
+// product of the stage's earlier radices); its twiddle for input r (1 <= r < R) is
+// tw[twoff + (r-1) * twns + kb + k * kmul], twns being the pass's global ns (the table row
+// length) and kb the sub-problem's base index (see StageArgs). twoff < 0: untwiddled (the
+// first pass of a plan). m_* are ceil(2^32 / d) for d = L/R, nsl, G (0 when d = 1).
+struct PassArgs {
+  int R, nsl;
+  uint32_t m_LR, m_nsl, m_G;
+  int64_t twoff, twns, kmul;
+};
+
+// One kernel stage: consecutive Stockham passes applied to independent sub-problems of
+// length L, G sub-problems per workgroup, grid (ceil(nsub / G), transforms).
 //
-// Sub-problem s (0 <= s < nsub) is (h, u) = (s / U, s % U): transform h of the batch,
-// part u of it. Element i (< L) of the sub-problem is read from
-//     in [h * idist + u * in_u + i * in_i]
-// and, after the stage's passes (the Stockham passes of the full plan restricted to
-// this sub-problem), element i is written to
-//     out[h * odist + u * out_u + i * out_i].
-// Pass p of the stage has radix R[p] and local ns nsl[p] (the product of the stage's
-// earlier radices). Its twiddle for butterfly input r (1 <= r < R) and local index k
-// (= j mod nsl) is tw[twoff[p] + (r-1) * twns[p] + kb + k * kmul] with kb = u * kb_u:
-// twns[p] is the pass's global ns (the table row length) and kmul, kb map the local
-// index to the global one. twoff[p] < 0 marks the untwiddled first pass of a plan.
+// Element e of sub-problem g (global index within its transform) of transform h is read
+// from in[h * in_h + g * in_g + e * in_e] and, after the passes, written to
+// out[h * out_h + g * out_g + e * out_e]. Twiddle base kb = g * kb_u. Optional real
+// weights: pre[g * pre_g + e * pre_e] multiplies the input element, post[...] the output.
 struct StageArgs {
-  int L;
-  int npass;
-  int R[MAX_PASSES];
-  int nsl[MAX_PASSES];
-  int64_t twoff[MAX_PASSES];
-  int64_t twns[MAX_PASSES];
-  int64_t kmul;
-  int64_t kb_u;
-  int64_t U;
-  int64_t in_u, in_i, out_u, out_i;
-  int64_t idist, odist;
+  int L, npass, G;
+  int gfast_in, gfast_out;  // 1: consecutive threads walk g (g adjacent in memory)
+  PassArgs pass[MAX_PASSES];
   int64_t nsub;
-  int G;        // sub-problems per workgroup
-  int gfast_in;  // 1: consecutive threads walk the sub-problem index on load
-  int gfast_out; // same for the store
-  // optional real weights (nullptr = none): input element i of every sub-problem is
-  // multiplied by pre[pre_off(u, i)] before the first pass, output element i by
-  // post[...] after the last; offsets use the same (u, i) -> element map as the data
-  // within one transform: e = u * in_u' + i * in_i' with the strides below.
+  int64_t in_g, in_e, in_h, out_g, out_e, out_h;
+  int64_t kb_u;
   const double *pre;
   const double *post;
-  int64_t pre_u, pre_i, post_u, post_i;
+  int64_t pre_g, pre_e, post_g, post_e;
 };
 
 // A size class of a ragged batch: the stages of one plan (at most two), with pass
@@ -64,18 +55,19 @@ struct StageArgs {
 struct RClass {
   int N, nstage, P, Q;
   int L[2], npass[2], G[2];
-  int R[2][MAX_PASSES];
-  int nsl[2][MAX_PASSES];
-  int64_t twoff[2][MAX_PASSES];
-  int64_t twns[2][MAX_PASSES];
+  PassArgs pass[2][MAX_PASSES];
 };
 
 // Describe a single-transform plan as a ragged size class, appending its twiddle table
 // to `tw` (offsets in the class are relative to the start of `tw`). Returns 0 on success.
 int export_class(const struct ::mfft_plan *p, RClass *c, std::vector<double2> *tw);
 
-// Launch one stage; returns a hipError_t.
-hipError_t launch_stage(bool fwd, const StageArgs &a, const double2 *in, double2 *out,
-                        const double2 *tw, hipStream_t stream);
+// Launch one stage over `ntrans` transforms; returns a hipError_t.
+hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,
+                        double2 *out, const double2 *tw, hipStream_t stream);
+
+inline uint32_t magic32(int64_t d) {
+  return d <= 1 ? 0u : (uint32_t)(((uint64_t(1) << 32) + (uint64_t)d - 1) / (uint64_t)d);
+}
 
 }  // namespace mfft

@@ -160,8 +160,20 @@ hipError_t build_twiddles(mfft_plan *p) {
   // stamp the offsets into the stages
   size_t t = 0;
   for (auto &st : p->stages)
-    for (int q = 0; q < st.npass; ++q, ++t) st.twoff[q] = off[t];
+    for (int q = 0; q < st.npass; ++q, ++t) st.pass[q].twoff = off[t];
   return hipSuccess;
+}
+
+void fill_pass(mfft::PassArgs &pa, int R, int64_t nsl, int64_t twns, int64_t kmul, int L,
+               int G) {
+  pa.R = R;
+  pa.nsl = (int)nsl;
+  pa.m_LR = mfft::magic32(L / R);
+  pa.m_nsl = mfft::magic32(nsl);
+  pa.m_G = mfft::magic32(G);
+  pa.twoff = -1;
+  pa.twns = twns;
+  pa.kmul = kmul;
 }
 
 // Build the stage descriptors of a plan whose radices and split are set.
@@ -178,28 +190,22 @@ bool build_stages(mfft_plan *p, int split) {
     StageArgs a{};
     a.L = (int)N;
     a.npass = s;
-    for (int t = 0; t < s; ++t) {
-      a.R[t] = rs[t];
-      a.nsl[t] = (int)ns[t];
-      a.twns[t] = ns[t];
-    }
-    a.kmul = 1;
-    a.kb_u = 0;
-    a.U = 1;
-    a.in_u = 0;
-    a.in_i = p->istride;
-    a.out_u = 0;
-    a.out_i = p->ostride;
-    a.idist = p->idist;
-    a.odist = p->odist;
-    a.nsub = p->howmany;
     a.G = (int)std::max<int64_t>(1, std::min<int64_t>(mfft::LDS_ELEMS / N, p->howmany));
+    for (int t = 0; t < s; ++t) fill_pass(a.pass[t], rs[t], ns[t], ns[t], 1, a.L, a.G);
+    a.nsub = p->howmany;
+    a.in_g = p->idist;
+    a.in_e = p->istride;
+    a.in_h = 0;
+    a.out_g = p->odist;
+    a.out_e = p->ostride;
+    a.out_h = 0;
+    a.kb_u = 0;
     a.gfast_in = std::llabs(p->idist) < std::llabs(p->istride) ? 1 : 0;
     a.gfast_out = std::llabs(p->odist) < std::llabs(p->ostride) ? 1 : 0;
-    a.pre_u = 0;
-    a.pre_i = 1;
-    a.post_u = 0;
-    a.post_i = 1;
+    a.pre_g = 0;
+    a.pre_e = 1;
+    a.post_g = 0;
+    a.post_e = 1;
     p->stages.push_back(a);
     p->stage_passes.push_back(s);
     p->chunk_h = p->howmany;
@@ -212,56 +218,44 @@ bool build_stages(mfft_plan *p, int split) {
   StageArgs A{}, B{};
   A.L = (int)P;
   A.npass = split;
-  for (int t = 0; t < split; ++t) {
-    A.R[t] = rs[t];
-    A.nsl[t] = (int)ns[t];
-    A.twns[t] = ns[t];
-  }
-  A.kmul = 1;
+  A.G = (int)std::max<int64_t>(1, std::min<int64_t>(mfft::LDS_ELEMS / P, Q));
+  for (int t = 0; t < split; ++t) fill_pass(A.pass[t], rs[t], ns[t], ns[t], 1, A.L, A.G);
+  A.nsub = Q;
+  A.in_g = p->istride;  // column c starts at element c
+  A.in_e = p->istride * Q;
+  A.in_h = p->idist;
+  A.out_g = P;  // the contiguous intermediate: column c lands at c*P
+  A.out_e = 1;
+  A.out_h = N;
   A.kb_u = 0;
-  A.U = Q;
-  A.in_u = p->istride;
-  A.in_i = p->istride * Q;
-  A.idist = p->idist;
-  A.out_u = P;  // into the contiguous intermediate
-  A.out_i = 1;
-  A.odist = N;
-  A.G = (int)std::max<int64_t>(1, mfft::LDS_ELEMS / P);
   A.gfast_in = 1;
   A.gfast_out = 0;
-  A.pre_u = 1;
-  A.pre_i = Q;
-  A.post_u = 0;
-  A.post_i = 0;
+  A.pre_g = 1;
+  A.pre_e = Q;
 
   B.L = (int)Q;
   B.npass = s - split;
-  for (int t = split; t < s; ++t) {
-    B.R[t - split] = rs[t];
-    B.nsl[t - split] = (int)(ns[t] / P);
-    B.twns[t - split] = ns[t];
-  }
-  B.kmul = P;
+  B.G = (int)std::max<int64_t>(1, std::min<int64_t>(mfft::LDS_ELEMS / Q, P));
+  for (int t = split; t < s; ++t)
+    fill_pass(B.pass[t - split], rs[t], ns[t] / P, ns[t], P, B.L, B.G);
+  B.nsub = P;  // residue classes k1 + P i
+  B.in_g = 1;
+  B.in_e = P;
+  B.in_h = N;
+  B.out_g = p->ostride;
+  B.out_e = p->ostride * P;
+  B.out_h = p->odist;
   B.kb_u = 1;
-  B.U = P;
-  B.in_u = 1;
-  B.in_i = P;
-  B.idist = N;
-  B.out_u = p->ostride;
-  B.out_i = p->ostride * P;
-  B.odist = p->odist;
-  B.G = (int)std::max<int64_t>(1, mfft::LDS_ELEMS / Q);
   B.gfast_in = 1;
   B.gfast_out = 1;
-  B.pre_u = 0;
-  B.pre_i = 0;
-  B.post_u = 1;
-  B.post_i = P;
+  B.post_g = 1;
+  B.post_e = P;
   p->stages.push_back(A);
   p->stages.push_back(B);
   p->stage_passes.push_back(split);
   p->stage_passes.push_back(s - split);
   p->chunk_h = std::max<int64_t>(1, std::min<int64_t>(p->howmany, chunk_bytes() / (16 * N)));
+  p->chunk_h = std::min<int64_t>(p->chunk_h, 65535);  // grid.y
   return true;
 }
 
@@ -294,22 +288,20 @@ int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
     StageArgs a = p->stages[0];
     a.pre = pre;
     a.post = post;
-    return mfft::launch_stage(fwd, a, in, out, p->d_tw, st) == hipSuccess ? MFFT_SUCCESS
-                                                                         : MFFT_ERR_HIP;
+    return mfft::launch_stage(fwd, a, 1, in, out, p->d_tw, st) == hipSuccess ? MFFT_SUCCESS
+                                                                            : MFFT_ERR_HIP;
   }
-  const int64_t N = p->n;
   for (int64_t h0 = 0; h0 < p->howmany; h0 += p->chunk_h) {
     const int64_t hc = std::min(p->chunk_h, p->howmany - h0);
     StageArgs A = p->stages[0], B = p->stages[1];
-    A.nsub = hc * A.U;
-    B.nsub = hc * B.U;
     A.pre = pre;
     B.post = post;
-    if (mfft::launch_stage(fwd, A, in + h0 * p->idist, p->d_scratch, p->d_tw, st) != hipSuccess)
+    if (mfft::launch_stage(fwd, A, hc, in + h0 * p->idist, p->d_scratch, p->d_tw, st) !=
+        hipSuccess)
       return MFFT_ERR_HIP;
-    if (mfft::launch_stage(fwd, B, p->d_scratch, out + h0 * p->odist, p->d_tw, st) != hipSuccess)
+    if (mfft::launch_stage(fwd, B, hc, p->d_scratch, out + h0 * p->odist, p->d_tw, st) !=
+        hipSuccess)
       return MFFT_ERR_HIP;
-    (void)N;
   }
   return MFFT_SUCCESS;
 }
@@ -328,11 +320,13 @@ int mfft::export_class(const mfft_plan *p, mfft::RClass *c, std::vector<double2>
     c->npass[s] = a.npass;
     c->G[s] = a.G;
     for (int q = 0; q < a.npass; ++q) {
-      c->R[s][q] = a.R[q];
-      c->nsl[s][q] = a.nsl[q];
-      c->twoff[s][q] = a.twoff[q] >= 0 ? a.twoff[q] + base : -1;
-      c->twns[s][q] = a.twns[q];
+      c->pass[s][q] = a.pass[q];
+      c->pass[s][q].twoff = a.pass[q].twoff >= 0 ? a.pass[q].twoff + base : -1;
     }
+  }
+  if (c->nstage == 1) {  // a ragged tile holds LDS_ELEMS / N transforms of the class
+    c->G[0] = std::max(1, mfft::LDS_ELEMS / c->L[0]);
+    for (int q = 0; q < c->npass[0]; ++q) c->pass[0][q].m_G = mfft::magic32(c->G[0]);
   }
   c->P = c->nstage == 2 ? p->stages[0].L : 0;
   c->Q = c->nstage == 2 ? p->stages[1].L : 0;

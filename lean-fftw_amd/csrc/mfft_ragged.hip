@@ -64,7 +64,6 @@ __global__ __launch_bounds__(WG_THREADS) void ragged_kernel(
     const int st = t.stage;
     const int L = c.L[st];
     const int gcount = t.count;
-    const int total = gcount * L;
     if (st == 1) {
       if (tid == 0) {
         unsigned spins = 0;
@@ -80,77 +79,23 @@ __global__ __launch_bounds__(WG_THREADS) void ragged_kernel(
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // see stage A's stores
     }
-    // load
-    const double2 *src = st == 1 ? out : in;
-    for (int f = tid; f < total; f += WG_THREADS) {
-      int g, i;
-      int64_t addr;
-      if (c.nstage == 1) {
-        g = f / L;
-        i = f - g * L;
-        addr = offs[t.base + g] + i;
-      } else {
-        i = f / gcount;
-        g = f - i * gcount;
-        addr = t.base + (t.sub0 + g) + (int64_t)i * (st == 0 ? c.Q : c.P);
-      }
-      lds[g * L + i] = src[addr];
-    }
-    __syncthreads();
     double2 *const buf0 = lds, *const buf1 = lds + LDS_ELEMS;
-    const int64_t kmul = st == 1 ? c.P : 1, kb_u = st == 1 ? 1 : 0;
-    const int64_t U = st == 1 ? c.P : 1;
-    for (int p = 0; p < c.npass[st]; ++p) {
-      const int64_t off = c.twoff[st][p];
-      const bool twd = off >= 0;
-      const double2 *twp = tw + (twd ? off : 0);
-      const int nsl = c.nsl[st][p];
-      const int64_t twns = c.twns[st][p];
-#define MFFT_PASS(RR)                                                                        \
-  case RR:                                                                                   \
-    pass_lds<FWD, RR>((p & 1) ? buf1 : buf0, (p & 1) ? buf0 : buf1, tid, gcount, L, nsl, twp, twd, twns,   \
-                      kmul, kb_u, t.sub0, U);                                                \
-    break;
-      switch (c.R[st][p]) {
-        MFFT_PASS(2)
-        MFFT_PASS(3)
-        MFFT_PASS(4)
-        MFFT_PASS(5)
-        MFFT_PASS(6)
-        MFFT_PASS(7)
-        MFFT_PASS(8)
-        MFFT_PASS(9)
-        MFFT_PASS(10)
-        MFFT_PASS(11)
-        MFFT_PASS(12)
-        MFFT_PASS(13)
-        MFFT_PASS(14)
-        MFFT_PASS(15)
-        MFFT_PASS(16)
-        default:
-          break;
-      }
-#undef MFFT_PASS
+    HbmSide hin{}, hout{};
+    int gin = 0, gout = 0;
+    if (c.nstage == 1) {  // G transforms of this size, listed by offset
+      hin = HbmSide{const_cast<double2 *>(in), offs + t.base, 0, 1, nullptr, 0, 0};
+      hout = HbmSide{out, offs + t.base, 0, 1, nullptr, 0, 0};
+    } else if (st == 0) {  // columns c of x[c + m Q] -> intermediate at c*P + k in out
+      hin = HbmSide{const_cast<double2 *>(in) + t.base + t.sub0, nullptr, 1, c.Q, nullptr, 0, 0};
+      hout = HbmSide{out + t.base + t.sub0 * c.P, nullptr, c.P, 1, nullptr, 0, 0};
+      gin = 1;
+    } else {  // residue classes k1 + P i, in place in out
+      hin = HbmSide{out + t.base + t.sub0, nullptr, 1, c.P, nullptr, 0, 0};
+      hout = HbmSide{out + t.base + t.sub0, nullptr, 1, c.P, nullptr, 0, 0};
+      gin = gout = 1;
     }
-    // store
-    for (int f = tid; f < total; f += WG_THREADS) {
-      int g, i;
-      int64_t addr;
-      if (c.nstage == 1) {
-        g = f / L;
-        i = f - g * L;
-        addr = offs[t.base + g] + i;
-      } else if (st == 0) {
-        g = f / L;
-        i = f - g * L;
-        addr = t.base + (t.sub0 + g) * c.P + i;
-      } else {
-        i = f / gcount;
-        g = f - i * gcount;
-        addr = t.base + (t.sub0 + g) + (int64_t)i * c.P;
-      }
-      out[addr] = ((c.npass[st] & 1) ? buf1 : buf0)[g * L + i];
-    }
+    run_stage<FWD>(buf0, buf1, hin, hout, tid, WG_THREADS, c.G[st], gcount, L, c.npass[st],
+                   c.pass[st], gin, gout, tw, t.sub0, st == 1 ? 1 : 0);
     if (c.nstage == 2 && st == 0) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // publish this tile's stores
       __syncthreads();
