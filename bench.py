@@ -116,6 +116,8 @@ def cpu_baseline_sweep(sizes, plans_radices, seconds):
 
 def bench_sweep(args, torch, lf):
     sizes = [1 << k for k in range(1, 19)]
+    if args.sizes:
+        sizes = [n for n in sizes if n in args.sizes]
     total = 1 << 24
     x = uniform_complex(torch, (total,), 20261203)
     y = torch.empty_like(x)
@@ -310,13 +312,30 @@ def bench_autograd(args, torch, lf):
                        "fwd_fft_GB/s": round(b / t_f / 1e9, 1)}}
 
 
+def bench_copy(args, torch, lf):
+    """A plain device copy of 2^24 complex128 (256 MB read + 256 MB written): the practical
+    HBM ceiling, and the reference launch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE."""
+    total = 1 << 24
+    x = uniform_complex(torch, (total,), 3)
+    y = torch.empty_like(x)
+    t = time_steps(torch, lambda: y.copy_(x), args.steps * 4, args.warmup)
+    b = 32.0 * total
+    return {"metric": "device copy GB/s", "value": round(b / t / 1e9, 1), "unit": "GB/s",
+            "n_gpus": 1, "steps": args.steps * 4, "warmup": args.warmup,
+            "ms_per_step": round(t * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "torch copy_ of 2^24 complex128 (256 MB in, 256 MB out)"},
+            "frac_of_peak": round(b / t / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="sweep",
-                    choices=["sweep", "single", "mixed", "ragged", "autograd"])
+                    choices=["sweep", "single", "mixed", "ragged", "autograd", "copy"])
+    ap.add_argument("--sizes", type=int, nargs="*", help="restrict the sweep to these sizes")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
@@ -328,7 +347,7 @@ def main():
 
     lf = load_package()
     fn = {"sweep": bench_sweep, "single": bench_single, "mixed": bench_mixed,
-          "ragged": bench_ragged, "autograd": bench_autograd}[args.config]
+          "ragged": bench_ragged, "autograd": bench_autograd, "copy": bench_copy}[args.config]
     res = fn(args, torch, lf)
     print(json.dumps(res), flush=True)
 

@@ -101,6 +101,10 @@ mfft_plan *mfft_plan_with_radices(int64_t n, int sign, int64_t howmany, const in
  * (it has no batched fast path, bench-leanfftw.c:257). */
 mfft_ragged *mfft_plan_ragged(int64_t count, const int64_t *sizes, int sign, unsigned flags);
 int mfft_execute_ragged(const mfft_ragged *p, const void *in, void *out, void *stream);
+/* The schedule a ragged batch uses for its transforms of size n: the radix sequence (return
+ * value: its length, -1 if n is not in the batch) and the number of passes of the first
+ * kernel stage (*split, 0 for a single stage). */
+int mfft_ragged_class_plan(const mfft_ragged *p, int64_t n, int *radices, int max, int *split);
 /* Synchronise `stream` and report whether the last ragged launch completed every tile
  * (MFFT_SUCCESS) or hit the bound of a stage-B wait (MFFT_ERR_HIP). */
 int mfft_ragged_status(const mfft_ragged *p, void *stream);

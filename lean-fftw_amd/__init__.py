@@ -67,6 +67,8 @@ def lib():
                 "mfft_plan_ragged": (vp, [i64, ctypes.POINTER(ctypes.c_int64), ctypes.c_int, ctypes.c_uint]),
                 "mfft_execute_ragged": (ctypes.c_int, [vp, vp, vp, vp]),
                 "mfft_ragged_status": (ctypes.c_int, [vp, vp]),
+                "mfft_ragged_class_plan": (ctypes.c_int, [vp, i64, ctypes.POINTER(ctypes.c_int),
+                                                          ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
                 "mfft_destroy_ragged": (None, [vp]),
                 "mfft_cos_sin_2pi": (None, [ctypes.c_uint64, ctypes.c_uint64, dp, dp]),
                 "mfft_tan_2pi": (ctypes.c_double, [ctypes.c_uint64, ctypes.c_uint64]),
@@ -294,6 +296,15 @@ class RaggedPlan:
 
     def status(self, stream=None):
         _check(lib().mfft_ragged_status(self._h, _stream_ptr(stream)), "ragged launch")
+
+    def class_plan(self, n: int):
+        """(radices, split) of the schedule used for the batch's transforms of size n."""
+        buf = (ctypes.c_int * 64)()
+        split = ctypes.c_int()
+        k = lib().mfft_ragged_class_plan(self._h, n, buf, 64, ctypes.byref(split))
+        if k < 0:
+            raise KeyError(n)
+        return list(buf[:k]), split.value
 
 
 # ------------------------------------------------------------------------------ torch-level API

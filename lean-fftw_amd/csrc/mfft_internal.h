@@ -18,7 +18,15 @@ void cos_sin_2pi_table(uint64_t n, std::vector<double> &cs);  // 2n doubles: c0,
 // ---------------------------------------------------------------- schedule
 constexpr int MAX_PASSES = 12;      // passes per kernel stage
 constexpr int WG_THREADS = 256;     // threads per workgroup of the stage kernels
-constexpr int LDS_ELEMS = 2048;     // complex doubles per LDS buffer (two buffers: 64 KiB)
+constexpr int LDS_ELEMS = 2048;     // complex doubles per LDS buffer, before padding
+// upper bound of a workgroup's dynamic LDS: LDS_ELEMS elements padded by one element per 16
+// plus one per sub-problem (see mfft_pass.h); the launches request only what their stage
+// needs
+constexpr int kLdsBytes = (LDS_ELEMS + LDS_ELEMS / 16 + LDS_ELEMS / 2) * 16;
+// the large-tile variant: one transform of up to BIG_ELEMS points per workgroup, 16 points
+// per thread, radices 2, 4, 8, 16
+constexpr int BIG_ELEMS = 8192;
+constexpr int kLdsBytesBig = (BIG_ELEMS + BIG_ELEMS / 16 + 1) * 16;
 
 // One Stockham pass inside a kernel stage. Local index k of butterfly j is j mod nsl (nsl:
 // product of the stage's earlier radices); its twiddle for input r (1 <= r < R) is
@@ -40,6 +48,8 @@ struct PassArgs {
 // weights: pre[g * pre_g + e * pre_e] multiplies the input element, post[...] the output.
 struct StageArgs {
   int L, npass, G;
+  int threads;  // workgroup size
+  int big;      // 1: the large-tile variant (one sub-problem of up to BIG_ELEMS per workgroup)
   int gfast_in, gfast_out;  // 1: consecutive threads walk g (g adjacent in memory)
   PassArgs pass[MAX_PASSES];
   int64_t nsub;
@@ -71,3 +81,9 @@ inline uint32_t magic32(int64_t d) {
 }
 
 }  // namespace mfft
+
+// Plan without the C ABI's defaults: `allow_big` = false keeps every stage in the
+// 256-thread layout (ragged size classes).
+mfft_plan *plan_many_internal(int64_t n, int sign, int64_t howmany, int64_t istride,
+                              int64_t idist, int64_t ostride, int64_t odist, unsigned flags,
+                              bool allow_big);

@@ -11,11 +11,11 @@
 
 namespace mfft {
 
-template <bool FWD>
-__global__ __launch_bounds__(WG_THREADS) void stage_kernel(const double2 *in, double2 *out,
-                                                           const double2 *__restrict__ tw,
-                                                           const StageArgs a) {
-  extern __shared__ double2 lds[];  // two buffers of G*L elements
+template <bool FWD, int RSET, int EPT, int MAXT>
+__global__ __launch_bounds__(MAXT) void stage_kernel(const double2 *in, double2 *out,
+                                                     const double2 *__restrict__ tw,
+                                                     const StageArgs a) {
+  extern __shared__ double2 lds[];  // G padded sub-problems
   const int g0 = blockIdx.x * a.G;
   const int64_t h = blockIdx.y;
   const int gcount = (int)(a.nsub - g0 < a.G ? a.nsub - g0 : a.G);
@@ -23,7 +23,7 @@ __global__ __launch_bounds__(WG_THREADS) void stage_kernel(const double2 *in, do
               a.pre ? a.pre + g0 * a.pre_g : nullptr, a.pre_g, a.pre_e};
   HbmSide hout{out + h * a.out_h + g0 * a.out_g, nullptr, a.out_g, a.out_e,
                a.post ? a.post + g0 * a.post_g : nullptr, a.post_g, a.post_e};
-  run_stage<FWD>(lds, lds + a.G * a.L, hin, hout, threadIdx.x, WG_THREADS, a.G, gcount, a.L,
+  run_stage<FWD, RSET, EPT>(lds, hin, hout, threadIdx.x, blockDim.x, a.G, gcount, a.L,
                  a.npass, a.pass, a.gfast_in, a.gfast_out, tw, g0, a.kb_u);
 }
 
@@ -31,12 +31,39 @@ hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const doub
                         double2 *out, const double2 *tw, hipStream_t stream) {
   if (a.nsub <= 0 || ntrans <= 0) return hipSuccess;
   const int64_t nx = (a.nsub + a.G - 1) / a.G;
-  const size_t lds = 2 * (size_t)a.G * a.L * sizeof(double2);
+  const size_t lds = (size_t)stage_lds_elems(a.G, a.L, a.npass) * sizeof(double2);
   const dim3 grid((unsigned)nx, (unsigned)ntrans);
-  if (fwd)
-    hipLaunchKernelGGL(stage_kernel<true>, grid, dim3(WG_THREADS), lds, stream, in, out, tw, a);
+  // variants: <direction, radix set, elements per thread, max threads>
+#define MFFT_K(F, S, E, T) stage_kernel<F, S, E, T>
+  static const bool attr = [] {
+    const void *ks[] = {(const void *)MFFT_K(true, RSET_ALL, 8, 256),
+                        (const void *)MFFT_K(false, RSET_ALL, 8, 256),
+                        (const void *)MFFT_K(true, RSET_SMALL, 8, 256),
+                        (const void *)MFFT_K(false, RSET_SMALL, 8, 256),
+                        (const void *)MFFT_K(true, RSET_POW2, 16, 512),
+                        (const void *)MFFT_K(false, RSET_POW2, 16, 512)};
+    for (const void *k : ks)
+      (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytesBig);
+    return true;
+  }();
+  (void)attr;
+  bool small = true;
+  for (int p = 0; p < a.npass; ++p) small = small && small_radix(a.pass[p].R);
+  const dim3 block(a.threads);
+  if (a.big) {
+    if (fwd)
+      hipLaunchKernelGGL((MFFT_K(true, RSET_POW2, 16, 512)), grid, block, lds, stream, in, out, tw, a);
+    else
+      hipLaunchKernelGGL((MFFT_K(false, RSET_POW2, 16, 512)), grid, block, lds, stream, in, out, tw, a);
+  } else if (fwd && small)
+    hipLaunchKernelGGL((MFFT_K(true, RSET_SMALL, 8, 256)), grid, block, lds, stream, in, out, tw, a);
+  else if (fwd)
+    hipLaunchKernelGGL((MFFT_K(true, RSET_ALL, 8, 256)), grid, block, lds, stream, in, out, tw, a);
+  else if (small)
+    hipLaunchKernelGGL((MFFT_K(false, RSET_SMALL, 8, 256)), grid, block, lds, stream, in, out, tw, a);
   else
-    hipLaunchKernelGGL(stage_kernel<false>, grid, dim3(WG_THREADS), lds, stream, in, out, tw, a);
+    hipLaunchKernelGGL((MFFT_K(false, RSET_ALL, 8, 256)), grid, block, lds, stream, in, out, tw, a);
+#undef MFFT_K
   return hipGetLastError();
 }
 

@@ -4,14 +4,46 @@
 // Pass layout inside a sub-problem of length L (LeanFFTW/Gen/Stockham.lean:12-20 and
 // passWith, :88-98): butterfly j (< L/R) reads x[j + r L/R]; with k = j mod ns and
 // b = j / ns it writes output r to x'[b ns R + k + r ns]. The inputs of a pass come from
-// HBM (first pass of a stage) or from an LDS buffer, its outputs go to the other LDS
-// buffer or to HBM (last pass of a stage), so a thread holds one group of butterflies at a
-// time and the intermediate passes alternate between two LDS buffers.
+// HBM (first pass of a stage) or from the LDS buffer, its outputs go to the LDS buffer or to
+// HBM (last pass of a stage). Each thread reads all its butterflies' inputs before a barrier
+// and writes their outputs after it, so the intermediate passes run in place in one buffer.
 #pragma once
 #include "bfly_gen.h"
 #include "mfft_internal.h"
 
 namespace mfft {
+
+// LDS position of element e of a sub-problem: one padding element after every 16, so that
+// butterfly outputs written at a stride of R elements (R <= 16) by consecutive lanes fall
+// into different banks. A sub-problem of length L occupies lds_len(L) elements.
+__device__ __forceinline__ int lpad(int e) { return e + (e >> 4); }
+__host__ __device__ constexpr int lds_len(int L) { return L + (L >> 4) + 1; }
+
+// LDS of a stage: none for one pass (HBM to HBM), one buffer of G padded sub-problems
+// otherwise (the intermediate passes run in place).
+__host__ __device__ constexpr int stage_lds_elems(int G, int L, int npass) {
+  return npass <= 1 ? 0 : G * lds_len(L);
+}
+
+// x * w for a twiddle input, exactly as the reference's general product mulGen(w, x)
+// (LeanFFTW/Gen/Sym.lean:94-97): re = fma(w.re, x.re, -(w.im x.im)),
+// im = fma(w.re, x.im, w.im x.re). A twiddled butterfly is this product on inputs 1..R-1
+// followed by the untwiddled butterfly: no smart constructor or contraction of the
+// generator sees through the product, so both DAGs round identically.
+__device__ __forceinline__ double2 twmul(double2 w, double2 x) {
+  return make_double2(fma(w.x, x.x, -(w.y * x.y)), fma(w.x, x.y, w.y * x.x));
+}
+
+// Radix sets of the kernel variants: SMALL holds the radices whose butterflies need few
+// registers, so a stage that uses only them runs at higher occupancy; POW2 is the set of
+// the large-tile variant.
+enum RadixSet { RSET_ALL = 0, RSET_SMALL = 1, RSET_POW2 = 2 };
+__host__ __device__ constexpr bool small_radix(int r) {
+  return r == 2 || r == 3 || r == 4 || r == 5 || r == 6 || r == 8;
+}
+__host__ __device__ constexpr bool in_set(int set, int r) {
+  return set == RSET_ALL || (set == RSET_SMALL ? small_radix(r) : (r == 2 || r == 4 || r == 8 || r == 16));
+}
 
 // q = n / d for n * d < 2^32, with m = ceil(2^32 / d) (m = 0 encodes d = 1)
 __device__ __forceinline__ uint32_t fdiv(uint32_t n, uint32_t m) {
@@ -33,83 +65,47 @@ __device__ __forceinline__ int64_t hbm_off(const HbmSide &s, int g) {
   return s.list ? s.list[g] : (int64_t)g * s.gstride;
 }
 
-template <bool FWD, int R, bool SRC_HBM, bool DST_HBM>
-__device__ __forceinline__ void pass_run(const double2 *src_lds, double2 *dst_lds,
-                                         const HbmSide &hin, const HbmSide &hout, int tid,
-                                         int nthreads, int G, int gcount, int L,
+template <bool FWD, int R, bool SRC_HBM, bool DST_HBM, int EPT>
+__device__ __forceinline__ void pass_run(double2 *buf, const HbmSide &hin, const HbmSide &hout,
+                                         int tid, int nthreads, int G, int gcount, int L,
                                          const PassArgs &pa, bool gfast,
                                          const double2 *__restrict__ tw, int64_t kb0,
                                          int64_t kb_u) {
+  // every butterfly of the pass, over all threads: a thread holds NB of them in registers
+  // (reads, then barrier, then writes), so the pass runs in place in one LDS buffer. A
+  // workgroup holds at most EPT * nthreads elements.
+  constexpr int NB = (EPT + R - 1) / R;
   const int LR = L / R;
-  const int nb = G * LR;
-  // several butterflies per iteration when reading HBM, so that >= 8 loads are in flight
-  constexpr int UF = SRC_HBM ? (R >= 8 ? 1 : (8 + R - 1) / R) : 1;
+  const int nb_total = G * LR;
   const bool twd = pa.twoff >= 0;
   const double2 *twp = tw + (twd ? pa.twoff : 0);
-#pragma unroll 1
-  for (int b0 = tid; b0 < nb; b0 += nthreads * UF) {
-    double2 x[UF][R];
-    int gs[UF], js[UF];
-    bool ok[UF];
+  const int ll = lds_len(L);
+  double2 x[NB][R];
+  int gs[NB], js[NB];
+  bool ok[NB];
 #pragma unroll
-    for (int u = 0; u < UF; ++u) {
-      const int b = b0 + u * nthreads;
-      int g, j;
-      if (gfast) {
-        j = (int)fdiv((uint32_t)b, pa.m_G);
-        g = b - j * G;
-      } else {
-        g = (int)fdiv((uint32_t)b, pa.m_LR);
-        j = b - g * LR;
-      }
-      gs[u] = g;
-      js[u] = j;
-      ok[u] = b < nb && g < gcount;
-      if (ok[u]) {
-        if constexpr (SRC_HBM) {
-          const double2 *s = hin.base + hbm_off(hin, g) + (int64_t)j * hin.stride;
-          const int64_t st = (int64_t)LR * hin.stride;
-#pragma unroll
-          for (int r = 0; r < R; ++r) x[u][r] = s[r * st];
-          if (hin.wt) {
-            const double *w = hin.wt + (int64_t)g * hin.wg + (int64_t)j * hin.wstride;
-            const int64_t ws = (int64_t)LR * hin.wstride;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-              const double v = w[r * ws];
-              x[u][r].x = x[u][r].x * v;
-              x[u][r].y = x[u][r].y * v;
-            }
-          }
-        } else {
-          const double2 *s = src_lds + g * L + j;
-#pragma unroll
-          for (int r = 0; r < R; ++r) x[u][r] = s[r * LR];
-        }
-      }
+  for (int u = 0; u < NB; ++u) {
+    const int b = tid + u * nthreads;
+    int g, j;
+    if (gfast) {
+      j = (int)fdiv((uint32_t)b, pa.m_G);
+      g = b - j * G;
+    } else {
+      g = (int)fdiv((uint32_t)b, pa.m_LR);
+      j = b - g * LR;
     }
+    gs[u] = g;
+    js[u] = j;
+    ok[u] = b < nb_total && g < gcount;
+    if (ok[u]) {
+      if constexpr (SRC_HBM) {
+        const double2 *s = hin.base + hbm_off(hin, g) + (int64_t)j * hin.stride;
+        const int64_t st = (int64_t)LR * hin.stride;
 #pragma unroll
-    for (int u = 0; u < UF; ++u) {
-      if (!ok[u]) continue;
-      const int g = gs[u], j = js[u];
-      const int bb = (int)fdiv((uint32_t)j, pa.m_nsl);
-      const int k = j - bb * pa.nsl;
-      if (twd) {
-        const int64_t kg = (kb0 + g) * kb_u + (int64_t)k * pa.kmul;
-        double2 w[R - 1];
-#pragma unroll
-        for (int r = 1; r < R; ++r) w[r - 1] = twp[(r - 1) * pa.twns + kg];
-        Bfly<FWD, R, true>::run(x[u], w);
-      } else {
-        Bfly<FWD, R, false>::run(x[u]);
-      }
-      const int e0 = bb * pa.nsl * R + k;
-      if constexpr (DST_HBM) {
-        double2 *d = hout.base + hbm_off(hout, g) + (int64_t)e0 * hout.stride;
-        const int64_t st = (int64_t)pa.nsl * hout.stride;
-        if (hout.wt) {
-          const double *w = hout.wt + (int64_t)g * hout.wg + (int64_t)e0 * hout.wstride;
-          const int64_t ws = (int64_t)pa.nsl * hout.wstride;
+        for (int r = 0; r < R; ++r) x[u][r] = s[r * st];
+        if (hin.wt) {
+          const double *w = hin.wt + (int64_t)g * hin.wg + (int64_t)j * hin.wstride;
+          const int64_t ws = (int64_t)LR * hin.wstride;
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             const double v = w[r * ws];
@@ -117,13 +113,46 @@ __device__ __forceinline__ void pass_run(const double2 *src_lds, double2 *dst_ld
             x[u][r].y = x[u][r].y * v;
           }
         }
-#pragma unroll
-        for (int r = 0; r < R; ++r) d[r * st] = x[u][r];
       } else {
-        double2 *d = dst_lds + g * L + e0;
+        const double2 *s = buf + g * ll;
 #pragma unroll
-        for (int r = 0; r < R; ++r) d[r * pa.nsl] = x[u][r];
+        for (int r = 0; r < R; ++r) x[u][r] = s[lpad(j + r * LR)];
       }
+    }
+  }
+  if constexpr (!SRC_HBM && !DST_HBM) __syncthreads();  // all reads before the in-place writes
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    if (!ok[u]) continue;
+    const int g = gs[u], j = js[u];
+    const int bb = (int)fdiv((uint32_t)j, pa.m_nsl);
+    const int k = j - bb * pa.nsl;
+    if (twd) {
+      const double2 *tp = twp + (kb0 + g) * kb_u + (int64_t)k * pa.kmul;
+#pragma unroll
+      for (int r = 1; r < R; ++r) x[u][r] = twmul(tp[(r - 1) * pa.twns], x[u][r]);
+    }
+    Bfly<FWD, R, false>::run(x[u]);
+    const int e0 = bb * pa.nsl * R + k;
+    if constexpr (DST_HBM) {
+      double2 *d = hout.base + hbm_off(hout, g) + (int64_t)e0 * hout.stride;
+      const int64_t st = (int64_t)pa.nsl * hout.stride;
+      if (hout.wt) {
+        const double *w = hout.wt + (int64_t)g * hout.wg + (int64_t)e0 * hout.wstride;
+        const int64_t ws = (int64_t)pa.nsl * hout.wstride;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const double v = w[r * ws];
+          x[u][r].x = x[u][r].x * v;
+          x[u][r].y = x[u][r].y * v;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r * st] = x[u][r];
+    } else {
+      double2 *d = buf + g * ll;
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[lpad(e0 + r * pa.nsl)] = x[u][r];
     }
   }
   if constexpr (!DST_HBM) __syncthreads();
@@ -131,33 +160,36 @@ __device__ __forceinline__ void pass_run(const double2 *src_lds, double2 *dst_ld
 
 // All passes of a stage: HBM -> (LDS ->)* HBM. Twiddle index of pass p for local index k
 // of sub-problem g: (kb0 + g) * kb_u + k * kmul.
-template <bool FWD>
-__device__ __forceinline__ void run_stage(double2 *buf0, double2 *buf1, const HbmSide &hin,
-                                          const HbmSide &hout, int tid, int nthreads, int G,
-                                          int gcount, int L, int npass, const PassArgs *pas,
-                                          int gfast_in, int gfast_out,
+template <bool FWD, int RSET, int EPT>
+__device__ __forceinline__ void run_stage(double2 *buf, const HbmSide &hin, const HbmSide &hout,
+                                          int tid, int nthreads, int G, int gcount, int L,
+                                          int npass,
+                                          const PassArgs *pas, int gfast_in, int gfast_out,
                                           const double2 *__restrict__ tw, int64_t kb0,
                                           int64_t kb_u) {
   for (int p = 0; p < npass; ++p) {
     const PassArgs pa = pas[p];
     const bool first = p == 0, last = p == npass - 1;
-    const double2 *src = (p & 1) ? buf1 : buf0;
-    double2 *dst = (p & 1) ? buf0 : buf1;
     const bool gfast = first ? gfast_in : (last ? gfast_out : false);
 #define MFFT_CASE(RR)                                                                            \
   case RR:                                                                                       \
+    if (!in_set(RSET, RR)) break;                                                                \
     if (first && last)                                                                           \
-      pass_run<FWD, RR, true, true>(src, dst, hin, hout, tid, nthreads, G, gcount, L, pa, gfast, \
-                                    tw, kb0, kb_u);                                             \
+      pass_run<FWD, RR, true, true, EPT>(buf, hin, hout, tid, nthreads, G, gcount, L, \
+        pa, gfast, tw,  \
+                                     kb0, kb_u);                                             \
     else if (first)                                                                              \
-      pass_run<FWD, RR, true, false>(src, dst, hin, hout, tid, nthreads, G, gcount, L, pa,      \
-                                     gfast, tw, kb0, kb_u);                                     \
+      pass_run<FWD, RR, true, false, EPT>(buf, hin, hout, tid, nthreads, G, gcount, L, \
+        pa, gfast, tw,  \
+                                     kb0, kb_u);                                     \
     else if (last)                                                                               \
-      pass_run<FWD, RR, false, true>(src, dst, hin, hout, tid, nthreads, G, gcount, L, pa,      \
-                                     gfast, tw, kb0, kb_u);                                     \
+      pass_run<FWD, RR, false, true, EPT>(buf, hin, hout, tid, nthreads, G, gcount, L, \
+        pa, gfast, tw,  \
+                                     kb0, kb_u);                                     \
     else                                                                                         \
-      pass_run<FWD, RR, false, false>(src, dst, hin, hout, tid, nthreads, G, gcount, L, pa,     \
-                                      gfast, tw, kb0, kb_u);                                    \
+      pass_run<FWD, RR, false, false, EPT>(buf, hin, hout, tid, nthreads, G, gcount, L, \
+        pa, gfast, tw,  \
+                                     kb0, kb_u);                                    \
     break;
     switch (pa.R) {
       MFFT_CASE(2)

@@ -48,6 +48,7 @@ struct mfft_plan {
   int64_t chunk_h = 0;
   int device = 0;
   int64_t spec_in_off = 0, spec_out_off = 0;  // doubles, for mfft_plan_spec plans
+  bool allow_big = true;  // may use the large-tile single-stage variant
 };
 
 namespace {
@@ -61,7 +62,7 @@ std::string g_wisdom_path;
 int64_t chunk_bytes() {
   static const int64_t v = [] {
     const char *e = std::getenv("MFFT_CHUNK_MB");
-    const int64_t mb = e ? std::atoll(e) : 64;
+    const int64_t mb = e ? std::atoll(e) : 256;
     return std::max<int64_t>(1, mb) << 20;
   }();
   return v;
@@ -186,11 +187,16 @@ bool build_stages(mfft_plan *p, int split) {
   std::vector<int64_t> ns(s + 1, 1);
   for (int t = 0; t < s; ++t) ns[t + 1] = ns[t] * rs[t];
   if (split <= 0 || split >= s) {
-    if (N > mfft::LDS_ELEMS || s > mfft::MAX_PASSES) return false;
+    bool pow2 = true;
+    for (int r : rs) pow2 = pow2 && (r == 2 || r == 4 || r == 8 || r == 16);
+    const bool big = N > mfft::LDS_ELEMS && N <= mfft::BIG_ELEMS && pow2 && p->allow_big;
+    if ((N > mfft::LDS_ELEMS && !big) || s > mfft::MAX_PASSES) return false;
     StageArgs a{};
     a.L = (int)N;
     a.npass = s;
-    a.G = (int)std::max<int64_t>(1, std::min<int64_t>(mfft::LDS_ELEMS / N, p->howmany));
+    a.big = big ? 1 : 0;
+    a.threads = big ? (int)(N / 16) : mfft::WG_THREADS;
+    a.G = big ? 1 : (int)std::max<int64_t>(1, std::min<int64_t>(mfft::LDS_ELEMS / N, p->howmany));
     for (int t = 0; t < s; ++t) fill_pass(a.pass[t], rs[t], ns[t], ns[t], 1, a.L, a.G);
     a.nsub = p->howmany;
     a.in_g = p->idist;
@@ -216,6 +222,7 @@ bool build_stages(mfft_plan *p, int split) {
       s - split > mfft::MAX_PASSES)
     return false;
   StageArgs A{}, B{};
+  A.threads = B.threads = mfft::WG_THREADS;
   A.L = (int)P;
   A.npass = split;
   A.G = (int)std::max<int64_t>(1, std::min<int64_t>(mfft::LDS_ELEMS / P, Q));
@@ -308,8 +315,42 @@ int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
 
 }  // namespace
 
+mfft_plan *plan_many_internal(int64_t n, int sign, int64_t howmany, int64_t istride,
+                              int64_t idist, int64_t ostride, int64_t odist, unsigned flags,
+                              bool allow_big) {
+  (void)flags;
+  if (!valid_problem(n, sign, howmany) || istride == 0 || ostride == 0) return nullptr;
+  auto *p = new mfft_plan;
+  p->n = n;
+  p->sign = sign;
+  p->howmany = howmany;
+  p->istride = istride;
+  p->idist = idist;
+  p->ostride = ostride;
+  p->odist = odist;
+  p->allow_big = allow_big;
+  const bool pow2 = (n & (n - 1)) == 0;
+  int split = 0;
+  if (n <= mfft::LDS_ELEMS || (allow_big && pow2 && n <= mfft::BIG_ELEMS)) {
+    p->radices = choose_radices(n);
+  } else {
+    int64_t P;
+    if (!choose_split(n, P)) {
+      delete p;
+      return nullptr;
+    }
+    std::vector<int> a = choose_radices(P), b = choose_radices(n / P);
+    p->radices = a;
+    p->radices.insert(p->radices.end(), b.begin(), b.end());
+    split = (int)a.size();
+  }
+  return finish(p, split);
+}
+
 int mfft::export_class(const mfft_plan *p, mfft::RClass *c, std::vector<double2> *tw) {
   if (!p || p->howmany != 1 || p->stages.empty() || p->stages.size() > 2) return 1;
+  for (const auto &st : p->stages)
+    if (st.big) return 1;  // the ragged kernel runs the 256-thread layout only
   const int64_t base = (int64_t)tw->size();
   *c = mfft::RClass{};
   c->N = (int)p->n;
@@ -357,31 +398,7 @@ mfft_plan *mfft_plan_with_radices(int64_t n, int sign, int64_t howmany, const in
 
 mfft_plan *mfft_plan_many(int64_t n, int sign, int64_t howmany, int64_t istride, int64_t idist,
                           int64_t ostride, int64_t odist, unsigned flags) {
-  (void)flags;
-  if (!valid_problem(n, sign, howmany) || istride == 0 || ostride == 0) return nullptr;
-  auto *p = new mfft_plan;
-  p->n = n;
-  p->sign = sign;
-  p->howmany = howmany;
-  p->istride = istride;
-  p->idist = idist;
-  p->ostride = ostride;
-  p->odist = odist;
-  int split = 0;
-  if (n <= mfft::LDS_ELEMS) {
-    p->radices = choose_radices(n);
-  } else {
-    int64_t P;
-    if (!choose_split(n, P)) {
-      delete p;
-      return nullptr;
-    }
-    std::vector<int> a = choose_radices(P), b = choose_radices(n / P);
-    p->radices = a;
-    p->radices.insert(p->radices.end(), b.begin(), b.end());
-    split = (int)a.size();
-  }
-  return finish(p, split);
+  return plan_many_internal(n, sign, howmany, istride, idist, ostride, odist, flags, true);
 }
 
 mfft_plan *mfft_plan_dft_1d(int64_t n, int sign, int64_t howmany, unsigned flags) {

@@ -9,21 +9,23 @@
 //   * a two-stage class (N = P * Q): stage-A tiles of G_A columns, stage-B tiles of G_B
 //     residue classes, as in the uniform path; stage A writes its intermediate into the
 //     output buffer and stage B transforms it there in place.
-// Tiles are queued largest class first (longest-processing-time order). The stage-B tiles
-// of a transform are queued a few transforms after its stage-A tiles, so they rarely wait,
-// and its intermediate is still in the Infinity Cache when they run.
+// Tiles are queued largest class first (longest-processing-time order); workgroups take
+// them from the queue with an agent-scope atomic counter until it runs out.
 //
-// Workgroups take tiles from the queue with an agent-scope atomic counter until it runs
-// out. A stage-B tile waits (bounded spin) until the transform's stage-A counter, raised
-// with release semantics after each stage-A tile, reaches the number of stage-A tiles;
-// stage-A tiles are always queued before the stage-B tiles of their transform and never
-// wait, so every wait ends. If a spin bound is ever hit the launch records an error flag
-// and still exits.
+// Default ("split") schedule: one persistent launch runs every stage-A tile and every
+// single-stage tile, a second runs every stage-B tile; the kernel boundary orders the
+// stages. Alternative ("one", MFFT_RAGGED_MODE=one): a single launch in which the stage-B
+// tiles of a transform are queued a few hundred tiles after its stage-A tiles and wait
+// (bounded spin) until the transform's stage-A counter, raised with release semantics after
+// each stage-A tile, reaches the number of stage-A tiles. Stage-A tiles are always queued
+// before the stage-B tiles of their transform and never wait, so every wait ends; if a spin
+// bound is ever hit the launch records an error flag and still exits.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <string>
 #include <vector>
 
 #include "../../include/mfft.h"
@@ -46,15 +48,16 @@ template <bool FWD>
 __global__ __launch_bounds__(WG_THREADS) void ragged_kernel(
     const double2 *in, double2 *out, const double2 *__restrict__ tw,
     const RClass *__restrict__ classes, const RTile *__restrict__ tiles, int ntiles,
-    const int64_t *__restrict__ offs, unsigned *counters /* [0]: queue head, [1]: error,
-    [2 + t]: stage-A tiles done of transform t */, const unsigned *__restrict__ need) {
+    const int64_t *__restrict__ offs, unsigned *head, unsigned *err,
+    unsigned *deps /* stage-A tiles done, per two-stage transform */,
+    const unsigned *__restrict__ need,
+    int wait /* 0: the stages run as separate launches, no cross-workgroup waits */) {
   extern __shared__ double2 lds[];
   __shared__ int s_tile;
   const int tid = threadIdx.x;
   for (;;) {
     if (tid == 0)
-      s_tile = (int)__hip_atomic_fetch_add(&counters[0], 1u, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
+      s_tile = (int)__hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     const int it = s_tile;
     __syncthreads();
@@ -64,22 +67,21 @@ __global__ __launch_bounds__(WG_THREADS) void ragged_kernel(
     const int st = t.stage;
     const int L = c.L[st];
     const int gcount = t.count;
-    if (st == 1) {
+    if (st == 1 && wait) {
       if (tid == 0) {
         unsigned spins = 0;
-        while (__hip_atomic_load(&counters[2 + t.dep], __ATOMIC_ACQUIRE,
-                                 __HIP_MEMORY_SCOPE_AGENT) < need[t.dep]) {
+        while (__hip_atomic_load(&deps[t.dep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+               need[t.dep]) {
           __builtin_amdgcn_s_sleep(2);
           if (++spins >= kSpinLimit) {
-            __hip_atomic_store(&counters[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
           }
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // see stage A's stores
       }
       __syncthreads();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // see stage A's stores
     }
-    double2 *const buf0 = lds, *const buf1 = lds + LDS_ELEMS;
     HbmSide hin{}, hout{};
     int gin = 0, gout = 0;
     if (c.nstage == 1) {  // G transforms of this size, listed by offset
@@ -94,14 +96,15 @@ __global__ __launch_bounds__(WG_THREADS) void ragged_kernel(
       hout = HbmSide{out + t.base + t.sub0, nullptr, 1, c.P, nullptr, 0, 0};
       gin = gout = 1;
     }
-    run_stage<FWD>(buf0, buf1, hin, hout, tid, WG_THREADS, c.G[st], gcount, L, c.npass[st],
-                   c.pass[st], gin, gout, tw, t.sub0, st == 1 ? 1 : 0);
-    if (c.nstage == 2 && st == 0) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // publish this tile's stores
-      __syncthreads();
-      if (tid == 0)
-        __hip_atomic_fetch_add(&counters[2 + t.dep], 1u, __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
+    run_stage<FWD, RSET_ALL, LDS_ELEMS / WG_THREADS>(lds, hin, hout, tid, WG_THREADS, c.G[st],
+                                                     gcount, L, c.npass[st], c.pass[st], gin, gout,
+                                                     tw, t.sub0, st == 1 ? 1 : 0);
+    if (c.nstage == 2 && st == 0 && wait) {
+      __syncthreads();  // every thread's stores of the tile are issued
+      if (tid == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // publish them
+        __hip_atomic_fetch_add(&deps[t.dep], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
     __syncthreads();
   }
@@ -120,15 +123,20 @@ struct mfft_ragged {
   int sign = -1;
   int64_t count = 0, total = 0;
   int ntiles = 0, ntwo = 0;
+  int n1 = 0, n2 = 0;  // split mode: stage-A and single-stage tiles, then stage-B tiles
+  bool split = true;
   int grid = 0;
   size_t lds_bytes = 0;
   mfft::RClass *d_classes = nullptr;
-  mfft::RTile *d_tiles = nullptr;
+  mfft::RTile *d_tiles = nullptr;   // one-launch order
+  mfft::RTile *d_tiles1 = nullptr;  // split mode, first launch
+  mfft::RTile *d_tiles2 = nullptr;  // split mode, second launch
   int64_t *d_offs = nullptr;
   unsigned *d_counters = nullptr;
   unsigned *d_need = nullptr;
   double2 *d_tw = nullptr;
   std::vector<mfft_plan *> plans;  // one per class (twiddle layout source)
+  std::map<int64_t, std::pair<std::vector<int>, int>> class_plan;  // size -> (radices, split)
 };
 
 namespace {
@@ -137,6 +145,8 @@ void free_ragged(mfft_ragged *r) {
   if (!r) return;
   (void)hipFree(r->d_classes);
   (void)hipFree(r->d_tiles);
+  (void)hipFree(r->d_tiles1);
+  (void)hipFree(r->d_tiles2);
   (void)hipFree(r->d_offs);
   (void)hipFree(r->d_counters);
   (void)hipFree(r->d_need);
@@ -168,12 +178,18 @@ mfft_ragged *mfft_plan_ragged(int64_t count, const int64_t *sizes, int sign, uns
   std::vector<double2> tw;
   std::map<int64_t, int> cls_of;
   for (auto &kv : by_size) {
-    mfft_plan *p = mfft_plan_dft_1d(kv.first, sign, 1, flags);
+    mfft_plan *p = plan_many_internal(kv.first, sign, 1, 1, kv.first, 1, kv.first, flags, false);
     if (!p) {
       free_ragged(r);
       return nullptr;
     }
     r->plans.push_back(p);
+    {
+      int buf[64], st[4];
+      const int nr = mfft_plan_radices(p, buf, 64);
+      const int ns = mfft_plan_stages(p, st, 4);
+      r->class_plan[kv.first] = {std::vector<int>(buf, buf + nr), ns == 2 ? st[0] : 0};
+    }
     mfft::RClass c{};
     if (mfft::export_class(p, &c, &tw) != 0) {
       free_ragged(r);
@@ -239,18 +255,46 @@ mfft_ragged *mfft_plan_ragged(int64_t count, const int64_t *sizes, int sign, uns
       }
     }
   }
-  // interleave: stage-B tiles of transform i follow the stage-A tiles of transform i+LAG
-  const size_t LAG = 4;
+  // Interleave: the stage-B tiles of a transform are queued only after at least LAG
+  // further stage-A tiles, LAG a few times the number of resident workgroups, so that its
+  // stage-A tiles have almost always finished when a workgroup takes its stage-B tiles.
+  int dev0 = 0, ncu0 = 0;
+  (void)hipGetDevice(&dev0);
+  (void)hipDeviceGetAttribute(&ncu0, hipDeviceAttributeMultiprocessorCount, dev0);
+  const size_t LAG = (size_t)std::max(ncu0, 1) * 2 * 3;
   std::vector<mfft::RTile> order;
+  size_t pending = 0, a_since = 0;  // next transform whose B tiles are pending
   for (size_t i = 0; i < twos.size(); ++i) {
     order.insert(order.end(), twos[i].a.begin(), twos[i].a.end());
-    if (i >= LAG) order.insert(order.end(), twos[i - LAG].b.begin(), twos[i - LAG].b.end());
+    a_since += twos[i].a.size();
+    // release B tiles of earlier transforms once LAG A tiles were queued after them
+    while (pending < i) {
+      size_t after = 0;
+      for (size_t q = pending + 1; q <= i; ++q) after += twos[q].a.size();
+      if (after < LAG) break;
+      order.insert(order.end(), twos[pending].b.begin(), twos[pending].b.end());
+      ++pending;
+    }
   }
-  for (size_t i = twos.size() > LAG ? twos.size() - LAG : 0; i < twos.size(); ++i)
-    order.insert(order.end(), twos[i].b.begin(), twos[i].b.end());
+  // the single-stage tiles never wait: queue them before the remaining B tiles
   order.insert(order.end(), single.begin(), single.end());
+  for (; pending < twos.size(); ++pending)
+    order.insert(order.end(), twos[pending].b.begin(), twos[pending].b.end());
+  (void)a_since;
   r->ntiles = (int)order.size();
   r->ntwo = (int)need.size();
+  // split mode: all stage-A tiles (largest first), then the single-stage tiles; then, in a
+  // second launch, all stage-B tiles
+  std::vector<mfft::RTile> first, second;
+  for (auto &tt : twos) first.insert(first.end(), tt.a.begin(), tt.a.end());
+  first.insert(first.end(), single.begin(), single.end());
+  for (auto &tt : twos) second.insert(second.end(), tt.b.begin(), tt.b.end());
+  r->n1 = (int)first.size();
+  r->n2 = (int)second.size();
+  {
+    const char *e = std::getenv("MFFT_RAGGED_MODE");
+    r->split = !(e && std::string(e) == "one");
+  }
 
   auto up = [](void **dst, const void *src, size_t bytes) {
     if (bytes == 0) bytes = 16;
@@ -259,11 +303,13 @@ mfft_ragged *mfft_plan_ragged(int64_t count, const int64_t *sizes, int sign, uns
   };
   bool ok = up((void **)&r->d_classes, classes.data(), classes.size() * sizeof(mfft::RClass)) &&
             up((void **)&r->d_tiles, order.data(), order.size() * sizeof(mfft::RTile)) &&
+            up((void **)&r->d_tiles1, first.data(), first.size() * sizeof(mfft::RTile)) &&
+            up((void **)&r->d_tiles2, second.data(), second.size() * sizeof(mfft::RTile)) &&
             up((void **)&r->d_offs, tile_offs.empty() ? nullptr : tile_offs.data(),
                tile_offs.size() * sizeof(int64_t)) &&
             up((void **)&r->d_need, need.empty() ? nullptr : need.data(),
                need.size() * sizeof(unsigned)) &&
-            up((void **)&r->d_counters, nullptr, (2 + need.size()) * sizeof(unsigned)) &&
+            up((void **)&r->d_counters, nullptr, (3 + need.size()) * sizeof(unsigned)) &&
             up((void **)&r->d_tw, tw.empty() ? nullptr : tw.data(), tw.size() * sizeof(double2));
   if (!ok) {
     free_ragged(r);
@@ -274,9 +320,19 @@ mfft_ragged *mfft_plan_ragged(int64_t count, const int64_t *sizes, int sign, uns
   int dev = 0, ncu = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  r->lds_bytes = 2 * (size_t)mfft::LDS_ELEMS * sizeof(double2);
-  // two resident workgroups per CU with 64 KiB of LDS each
-  r->grid = std::max(1, std::min(r->ntiles, std::max(ncu, 1) * 2));
+  size_t need_lds = 0;
+  for (const auto &c : classes)
+    for (int s2 = 0; s2 < c.nstage; ++s2)
+      need_lds = std::max(need_lds, (size_t)mfft::stage_lds_elems(c.G[s2], c.L[s2], c.npass[s2]) *
+                                        sizeof(double2));
+  r->lds_bytes = need_lds;
+  (void)hipFuncSetAttribute((const void *)mfft::ragged_kernel<true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, mfft::kLdsBytes);
+  (void)hipFuncSetAttribute((const void *)mfft::ragged_kernel<false>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, mfft::kLdsBytes);
+  // a few resident workgroups per CU (about 35 KiB of LDS each); workgroups that do not fit
+  // start when others finish and take what is left of the queue
+  r->grid = std::max(1, std::min(r->ntiles, std::max(ncu, 1) * 4));
   return r;
 }
 
@@ -285,17 +341,29 @@ int mfft_execute_ragged(const mfft_ragged *r, const void *in, void *out, void *s
   if (in == out) return MFFT_ERR_UNSUPPORTED;  // stage A writes its intermediate into out
   if (r->ntiles == 0) return MFFT_SUCCESS;
   hipStream_t st = (hipStream_t)stream;
-  const int nc = 2 + r->ntwo;
-  hipLaunchKernelGGL(mfft::ragged_reset, dim3((nc + 255) / 256), dim3(256), 0, st, r->d_counters,
-                     nc);
-  if (r->sign < 0)
-    hipLaunchKernelGGL(mfft::ragged_kernel<true>, dim3(r->grid), dim3(mfft::WG_THREADS),
-                       r->lds_bytes, st, (const double2 *)in, (double2 *)out, r->d_tw,
-                       r->d_classes, r->d_tiles, r->ntiles, r->d_offs, r->d_counters, r->d_need);
-  else
-    hipLaunchKernelGGL(mfft::ragged_kernel<false>, dim3(r->grid), dim3(mfft::WG_THREADS),
-                       r->lds_bytes, st, (const double2 *)in, (double2 *)out, r->d_tw,
-                       r->d_classes, r->d_tiles, r->ntiles, r->d_offs, r->d_counters, r->d_need);
+  const int nc = 3 + r->ntwo;  // [0] head, [1] error, [2] second head, [3 + t] stage-A done
+  unsigned *ctr = r->d_counters;
+  hipLaunchKernelGGL(mfft::ragged_reset, dim3((nc + 255) / 256), dim3(256), 0, st, ctr, nc);
+  const double2 *x = (const double2 *)in;
+  double2 *y = (double2 *)out;
+  auto launch = [&](const mfft::RTile *tiles, int n, unsigned *head, int wait) {
+    if (n == 0) return;
+    const dim3 grid(std::max(1, std::min(n, r->grid))), block(mfft::WG_THREADS);
+    if (r->sign < 0)
+      hipLaunchKernelGGL(mfft::ragged_kernel<true>, grid, block, r->lds_bytes, st, x, y, r->d_tw,
+                         r->d_classes, tiles, n, r->d_offs, head, ctr + 1, ctr + 3, r->d_need,
+                         wait);
+    else
+      hipLaunchKernelGGL(mfft::ragged_kernel<false>, grid, block, r->lds_bytes, st, x, y, r->d_tw,
+                         r->d_classes, tiles, n, r->d_offs, head, ctr + 1, ctr + 3, r->d_need,
+                         wait);
+  };
+  if (r->split) {
+    launch(r->d_tiles1, r->n1, ctr, 0);
+    launch(r->d_tiles2, r->n2, ctr + 2, 0);
+  } else {
+    launch(r->d_tiles, r->ntiles, ctr, 1);
+  }
   return hipGetLastError() == hipSuccess ? MFFT_SUCCESS : MFFT_ERR_HIP;
 }
 
@@ -307,6 +375,16 @@ int mfft_ragged_status(const mfft_ragged *r, void *stream) {
       hipMemcpy(&flag, r->d_counters + 1, sizeof(unsigned), hipMemcpyDeviceToHost) != hipSuccess)
     return MFFT_ERR_HIP;
   return flag ? MFFT_ERR_HIP : MFFT_SUCCESS;
+}
+
+int mfft_ragged_class_plan(const mfft_ragged *r, int64_t n, int *radices, int max, int *split) {
+  if (!r) return -1;
+  auto it = r->class_plan.find(n);
+  if (it == r->class_plan.end()) return -1;
+  const auto &rs = it->second.first;
+  for (int i = 0; i < (int)rs.size() && i < max; ++i) radices[i] = rs[i];
+  if (split) *split = it->second.second;
+  return (int)rs.size();
 }
 
 void mfft_destroy_ragged(mfft_ragged *r) { free_ragged(r); }

@@ -30,7 +30,7 @@ def check_ragged(lf, oracle, sizes, sign, seed, reps=1):
     off = 0
     radices = {}
     for n in sorted(set(sizes)):
-        radices[n] = lf.Plan(n, sign, 1).radices
+        radices[n] = rp.class_plan(n)[0]
     bad = []
     for i, n in enumerate(sizes):
         ref = oracle.ref_transform(sign < 0, radices[n], x[off:off + n])
@@ -60,8 +60,8 @@ def test_ragged_benchmark_draw_prefix(lf, oracle):
 
 
 def test_ragged_full_draw_against_uniform(lf):
-    """The whole 10,000-transform draw: the persistent launch equals per-transform plain
-    launches of the uniform path, bit for bit."""
+    """The whole 10,000-transform draw: the persistent launches equal per-transform plain
+    launches of the uniform path with the same schedules, bit for bit."""
     w = load_product().workloads
     sizes = w.ragged_sizes()
     total = sum(sizes)
@@ -78,7 +78,8 @@ def test_ragged_full_draw_against_uniform(lf):
     for n in sizes:
         p = plans.get(n)
         if p is None:
-            p = plans[n] = lf.Plan(n, -1, 1)
+            rs, split = rp.class_plan(n)
+            p = plans[n] = lf.Plan(n, -1, 1, radices=rs, split=split)
         p.call(x[off:off + n], ref[off:off + n])
         off += n
     torch.cuda.synchronize()
