@@ -117,6 +117,14 @@ void mfft_destroy_ragged(mfft_ragged *p);
 int mfft_execute_weighted(const mfft_plan *p, const void *in, void *out, const double *pre,
                           const double *post, void *stream);
 
+/* As mfft_execute_weighted, and every workgroup of the last kernel stage also writes the sum
+ * of |out|^2 over the elements it stored to partials[i], 0 <= i < mfft_plan_partials(p):
+ * the sum of the partials is sum |out|^2 (a deterministic fixed-order reduction). For the
+ * loss of the differentiable path; no counterpart in the reference. */
+int64_t mfft_plan_partials(const mfft_plan *p);
+int mfft_execute_weighted_sumsq(const mfft_plan *p, const void *in, void *out, const double *pre,
+                                const double *post, double *partials, void *stream);
+
 /* The reference's twiddle routine (LeanFFTW/Core/Trig.lean:126, :90-92), exposed for
  * tests: (cos, sin)(2 pi k / n) and tan(2 pi num / den) as binary64. */
 void mfft_cos_sin_2pi(uint64_t k, uint64_t n, double *c, double *s);

@@ -61,6 +61,8 @@ def lib():
                 "mfft_execute_host": (ctypes.c_int, [vp, dp, dp]),
                 "mfft_execute_spec": (ctypes.c_int, [vp, vp, vp, vp]),
                 "mfft_execute_weighted": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
+                "mfft_execute_weighted_sumsq": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+                "mfft_plan_partials": (i64, [vp]),
                 "mfft_destroy_plan": (None, [vp]),
                 "mfft_plan_radices": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
                 "mfft_plan_stages": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
@@ -210,6 +212,26 @@ class Plan:
                                            _stream_ptr(stream)), "mfft_execute_weighted")
         return out
 
+    @property
+    def partials(self) -> int:
+        return int(lib().mfft_plan_partials(self._h))
+
+    def call_weighted_sumsq(self, inp, out, partials, pre=None, post=None, stream=None):
+        """As call_weighted; partials (float64, length self.partials) receives per-workgroup
+        sums of |out|^2, whose total is sum |out|^2."""
+        import torch
+
+        self._check_tensor(inp, "input")
+        self._check_tensor(out, "output")
+        if partials.dtype != torch.float64 or not partials.is_cuda or partials.numel() < self.partials:
+            raise TypeError("partials must be a float64 CUDA tensor of length plan.partials")
+        ptr = lambda w: ctypes.c_void_p(w.data_ptr()) if w is not None else None  # noqa: E731
+        _check(lib().mfft_execute_weighted_sumsq(self._h, ctypes.c_void_p(inp.data_ptr()),
+                                                 ctypes.c_void_p(out.data_ptr()), ptr(pre),
+                                                 ptr(post), ctypes.c_void_p(partials.data_ptr()),
+                                                 _stream_ptr(stream)), "mfft_execute_weighted_sumsq")
+        return out
+
     def exec(self, inp, stream=None):
         import torch
 
@@ -337,4 +359,4 @@ def ifft_unnormalized(x):
 
 
 from . import workloads  # noqa: E402,F401
-from .autograd import FFTFunction, fft_autograd, weighted_power_loss  # noqa: E402,F401
+from .autograd import FFTFunction, WeightedPowerLoss, fft_autograd, weighted_power_loss  # noqa: E402,F401

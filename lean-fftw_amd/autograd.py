@@ -9,7 +9,9 @@ are the forward ones without the negated imaginary part (LeanFFTW/Core/FastPlan.
 For real weights and PyTorch's conjugate-Wirtinger convention, the gradient of a real loss
 L(y) with respect to x is exactly that: grad_x = F^H (weight * grad_y).
 
-The weight is treated as a constant (no gradient flows to it).
+WeightedPowerLoss fuses the whole loss sum |w * F x|^2 (config of the differentiable
+benchmark) into one forward and one adjoint kernel. The weight is treated as a constant
+(no gradient flows to it).
 """
 
 from __future__ import annotations
@@ -72,7 +74,41 @@ def fft_autograd(x, weight=None, sign: int = -1):
     return FFTFunction.apply(x, weight, sign)
 
 
+class WeightedPowerLoss(torch.autograd.Function):
+    """loss = sum |w * F x|^2 over a batch of rows, fused.
+
+    Forward: one kernel writes y = w * F x and, in the same stores, per-workgroup partial
+    sums of |y|^2 (a fixed-order reduction, so the loss is deterministic); the loss is the
+    sum of those partials. Backward: grad_x = F^H (2 g w * y), g = d(out)/d(loss): one
+    adjoint kernel reading the saved y with the real weight 2 g w fused into its load.
+    """
+
+    @staticmethod
+    def forward(ctx, x, w):
+        if x.dtype != torch.complex128 or not x.is_cuda:
+            raise TypeError("WeightedPowerLoss takes complex128 CUDA tensors")
+        x = x.contiguous()
+        n = x.shape[-1]
+        howmany = x.numel() // n
+        w = w.to(device=x.device, dtype=torch.float64).contiguous()
+        plan = _plan(n, -1, howmany)
+        y = torch.empty_like(x)
+        parts = torch.empty(plan.partials, dtype=torch.float64, device=x.device)
+        plan.call_weighted_sumsq(x, y, parts, post=w)
+        ctx.save_for_backward(y, w)
+        ctx.meta = (n, howmany)
+        return parts.sum()
+
+    @staticmethod
+    def backward(ctx, g):
+        y, w = ctx.saved_tensors
+        n, howmany = ctx.meta
+        scale = (2.0 * g).to(torch.float64) * w
+        out = torch.empty_like(y)
+        _plan(n, 1, howmany).call_weighted(y, out, pre=scale)
+        return out, None
+
+
 def weighted_power_loss(x, w):
     """loss = sum(|w * FFT(x)|^2) along the last dimension, differentiable in x."""
-    y = FFTFunction.apply(x, w, -1)
-    return (y.real * y.real + y.imag * y.imag).sum()
+    return WeightedPowerLoss.apply(x, w)

@@ -58,6 +58,9 @@ struct StageArgs {
   const double *pre;
   const double *post;
   int64_t pre_g, pre_e, post_g, post_e;
+  // optional: each workgroup writes the sum of |out|^2 of its stores to
+  // sumsq[blockIdx.y * gridDim.x + blockIdx.x] (last stage of a plan only)
+  double *sumsq;
 };
 
 // A size class of a ragged batch: the stages of one plan (at most two), with pass

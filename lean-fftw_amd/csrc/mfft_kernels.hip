@@ -11,7 +11,7 @@
 
 namespace mfft {
 
-template <bool FWD, int RSET, int EPT, int MAXT>
+template <bool FWD, int RSET, int EPT, int MAXT, bool SQ>
 __global__ __launch_bounds__(MAXT) void stage_kernel(const double2 *in, double2 *out,
                                                      const double2 *__restrict__ tw,
                                                      const StageArgs a) {
@@ -23,8 +23,32 @@ __global__ __launch_bounds__(MAXT) void stage_kernel(const double2 *in, double2 
               a.pre ? a.pre + g0 * a.pre_g : nullptr, a.pre_g, a.pre_e};
   HbmSide hout{out + h * a.out_h + g0 * a.out_g, nullptr, a.out_g, a.out_e,
                a.post ? a.post + g0 * a.post_g : nullptr, a.post_g, a.post_e};
-  run_stage<FWD, RSET, EPT>(lds, hin, hout, threadIdx.x, blockDim.x, a.G, gcount, a.L,
+  const double part = run_stage<FWD, RSET, EPT, SQ>(lds, hin, hout, threadIdx.x, blockDim.x, a.G, gcount, a.L,
                  a.npass, a.pass, a.gfast_in, a.gfast_out, tw, g0, a.kb_u);
+  if constexpr (SQ) {
+    // deterministic per-workgroup partial of sum |out|^2: a fixed reduction tree
+    __shared__ double wsum[MAXT / 64];
+    double v = part;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t = t + wsum[w];
+      a.sumsq[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = t;
+    }
+  }
+}
+
+// the kernel instance for (direction, radix set, sum of squares)
+const void *kernel_for(int fwd, int set, int sq) {
+#define MFFT_K(F, S, E, T)                                                                   \
+  (sq ? (const void *)stage_kernel<F, S, E, T, true> : (const void *)stage_kernel<F, S, E, T, false>)
+  if (set == RSET_POW2) return fwd ? MFFT_K(true, RSET_POW2, 16, 512) : MFFT_K(false, RSET_POW2, 16, 512);
+  if (set == RSET_SMALL) return fwd ? MFFT_K(true, RSET_SMALL, 8, 256) : MFFT_K(false, RSET_SMALL, 8, 256);
+  return fwd ? MFFT_K(true, RSET_ALL, 8, 256) : MFFT_K(false, RSET_ALL, 8, 256);
+#undef MFFT_K
 }
 
 hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,
@@ -33,37 +57,22 @@ hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const doub
   const int64_t nx = (a.nsub + a.G - 1) / a.G;
   const size_t lds = (size_t)stage_lds_elems(a.G, a.L, a.npass) * sizeof(double2);
   const dim3 grid((unsigned)nx, (unsigned)ntrans);
-  // variants: <direction, radix set, elements per thread, max threads>
-#define MFFT_K(F, S, E, T) stage_kernel<F, S, E, T>
+  // variants: <direction, radix set, elements per thread, max threads, sum of squares>
+  bool small = true;
+  for (int p = 0; p < a.npass; ++p) small = small && small_radix(a.pass[p].R);
+  const int set = a.big ? RSET_POW2 : (small ? RSET_SMALL : RSET_ALL);
+  const void *k = kernel_for(fwd, set, a.sumsq != nullptr);
   static const bool attr = [] {
-    const void *ks[] = {(const void *)MFFT_K(true, RSET_ALL, 8, 256),
-                        (const void *)MFFT_K(false, RSET_ALL, 8, 256),
-                        (const void *)MFFT_K(true, RSET_SMALL, 8, 256),
-                        (const void *)MFFT_K(false, RSET_SMALL, 8, 256),
-                        (const void *)MFFT_K(true, RSET_POW2, 16, 512),
-                        (const void *)MFFT_K(false, RSET_POW2, 16, 512)};
-    for (const void *k : ks)
-      (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytesBig);
+    for (int f = 0; f < 2; ++f)
+      for (int st = 0; st < 3; ++st)
+        for (int q = 0; q < 2; ++q)
+          (void)hipFuncSetAttribute(kernel_for(f, st, q), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    kLdsBytesBig);
     return true;
   }();
   (void)attr;
-  bool small = true;
-  for (int p = 0; p < a.npass; ++p) small = small && small_radix(a.pass[p].R);
-  const dim3 block(a.threads);
-  if (a.big) {
-    if (fwd)
-      hipLaunchKernelGGL((MFFT_K(true, RSET_POW2, 16, 512)), grid, block, lds, stream, in, out, tw, a);
-    else
-      hipLaunchKernelGGL((MFFT_K(false, RSET_POW2, 16, 512)), grid, block, lds, stream, in, out, tw, a);
-  } else if (fwd && small)
-    hipLaunchKernelGGL((MFFT_K(true, RSET_SMALL, 8, 256)), grid, block, lds, stream, in, out, tw, a);
-  else if (fwd)
-    hipLaunchKernelGGL((MFFT_K(true, RSET_ALL, 8, 256)), grid, block, lds, stream, in, out, tw, a);
-  else if (small)
-    hipLaunchKernelGGL((MFFT_K(false, RSET_SMALL, 8, 256)), grid, block, lds, stream, in, out, tw, a);
-  else
-    hipLaunchKernelGGL((MFFT_K(false, RSET_ALL, 8, 256)), grid, block, lds, stream, in, out, tw, a);
-#undef MFFT_K
+  void *args[] = {(void *)&in, (void *)&out, (void *)&tw, (void *)&a};
+  (void)hipLaunchKernel(k, grid, dim3(a.threads), args, lds, stream);
   return hipGetLastError();
 }
 

@@ -52,7 +52,8 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, uint32_t m) {
 
 // Where a pass reads or writes in HBM: element e of sub-problem g of the tile is
 // base[off(g) + e * stride], off(g) = list ? list[g] : g * gstride, optionally scaled by a
-// real weight wt[g * wg + e * wstride] (weights are shared across the batch).
+// real weight wt[g * wg + e * wstride] (weights are shared across the batch). With SQ the
+// last pass also returns the sum of |x|^2 of the elements the thread stored.
 struct HbmSide {
   double2 *base;
   const int64_t *list;
@@ -65,8 +66,8 @@ __device__ __forceinline__ int64_t hbm_off(const HbmSide &s, int g) {
   return s.list ? s.list[g] : (int64_t)g * s.gstride;
 }
 
-template <bool FWD, int R, bool SRC_HBM, bool DST_HBM, int EPT>
-__device__ __forceinline__ void pass_run(double2 *buf, const HbmSide &hin, const HbmSide &hout,
+template <bool FWD, int R, bool SRC_HBM, bool DST_HBM, int EPT, bool SQ>
+__device__ __forceinline__ double pass_run(double2 *buf, const HbmSide &hin, const HbmSide &hout,
                                          int tid, int nthreads, int G, int gcount, int L,
                                          const PassArgs &pa, bool gfast,
                                          const double2 *__restrict__ tw, int64_t kb0,
@@ -121,6 +122,7 @@ __device__ __forceinline__ void pass_run(double2 *buf, const HbmSide &hin, const
     }
   }
   if constexpr (!SRC_HBM && !DST_HBM) __syncthreads();  // all reads before the in-place writes
+  double sumsq = 0.0;  // sum of |output|^2 over this thread's HBM stores, when requested
 #pragma unroll
   for (int u = 0; u < NB; ++u) {
     if (!ok[u]) continue;
@@ -149,6 +151,10 @@ __device__ __forceinline__ void pass_run(double2 *buf, const HbmSide &hin, const
       }
 #pragma unroll
       for (int r = 0; r < R; ++r) d[r * st] = x[u][r];
+      if constexpr (SQ) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) sumsq = sumsq + (x[u][r].x * x[u][r].x + x[u][r].y * x[u][r].y);
+      }
     } else {
       double2 *d = buf + g * ll;
 #pragma unroll
@@ -156,17 +162,19 @@ __device__ __forceinline__ void pass_run(double2 *buf, const HbmSide &hin, const
     }
   }
   if constexpr (!DST_HBM) __syncthreads();
+  return sumsq;
 }
 
 // All passes of a stage: HBM -> (LDS ->)* HBM. Twiddle index of pass p for local index k
 // of sub-problem g: (kb0 + g) * kb_u + k * kmul.
-template <bool FWD, int RSET, int EPT>
-__device__ __forceinline__ void run_stage(double2 *buf, const HbmSide &hin, const HbmSide &hout,
+template <bool FWD, int RSET, int EPT, bool SQ = false>
+__device__ __forceinline__ double run_stage(double2 *buf, const HbmSide &hin, const HbmSide &hout,
                                           int tid, int nthreads, int G, int gcount, int L,
                                           int npass,
                                           const PassArgs *pas, int gfast_in, int gfast_out,
                                           const double2 *__restrict__ tw, int64_t kb0,
                                           int64_t kb_u) {
+  double sumsq = 0.0;
   for (int p = 0; p < npass; ++p) {
     const PassArgs pa = pas[p];
     const bool first = p == 0, last = p == npass - 1;
@@ -175,19 +183,19 @@ __device__ __forceinline__ void run_stage(double2 *buf, const HbmSide &hin, cons
   case RR:                                                                                       \
     if (!in_set(RSET, RR)) break;                                                                \
     if (first && last)                                                                           \
-      pass_run<FWD, RR, true, true, EPT>(buf, hin, hout, tid, nthreads, G, gcount, L, \
+      sumsq = pass_run<FWD, RR, true, true, EPT, SQ>(buf, hin, hout, tid, nthreads, G, gcount, L, \
         pa, gfast, tw,  \
                                      kb0, kb_u);                                             \
     else if (first)                                                                              \
-      pass_run<FWD, RR, true, false, EPT>(buf, hin, hout, tid, nthreads, G, gcount, L, \
+      pass_run<FWD, RR, true, false, EPT, SQ>(buf, hin, hout, tid, nthreads, G, gcount, L, \
         pa, gfast, tw,  \
                                      kb0, kb_u);                                     \
     else if (last)                                                                               \
-      pass_run<FWD, RR, false, true, EPT>(buf, hin, hout, tid, nthreads, G, gcount, L, \
+      sumsq = pass_run<FWD, RR, false, true, EPT, SQ>(buf, hin, hout, tid, nthreads, G, gcount, L, \
         pa, gfast, tw,  \
                                      kb0, kb_u);                                     \
     else                                                                                         \
-      pass_run<FWD, RR, false, false, EPT>(buf, hin, hout, tid, nthreads, G, gcount, L, \
+      pass_run<FWD, RR, false, false, EPT, SQ>(buf, hin, hout, tid, nthreads, G, gcount, L, \
         pa, gfast, tw,  \
                                      kb0, kb_u);                                    \
     break;
@@ -212,6 +220,7 @@ __device__ __forceinline__ void run_stage(double2 *buf, const HbmSide &hin, cons
     }
 #undef MFFT_CASE
   }
+  return sumsq;
 }
 
 }  // namespace mfft

@@ -286,8 +286,16 @@ bool valid_problem(int64_t n, int sign, int64_t howmany) {
          howmany >= 0;
 }
 
+int64_t count_partials(const mfft_plan *p) {
+  if (p->howmany == 0 || p->stages.empty()) return 0;
+  const StageArgs &last = p->stages.back();
+  const int64_t nx = (last.nsub + last.G - 1) / last.G;
+  if (p->stages.size() == 1) return nx;
+  return nx * p->howmany;  // one row of nx workgroups per transform, over all chunks
+}
+
 int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
-        const double *post, hipStream_t st) {
+        const double *post, double *sumsq, hipStream_t st) {
   if (p->howmany == 0) return MFFT_SUCCESS;
   if (!in || !out) return MFFT_ERR_INVALID;
   const bool fwd = p->sign < 0;
@@ -295,14 +303,17 @@ int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
     StageArgs a = p->stages[0];
     a.pre = pre;
     a.post = post;
+    a.sumsq = sumsq;
     return mfft::launch_stage(fwd, a, 1, in, out, p->d_tw, st) == hipSuccess ? MFFT_SUCCESS
                                                                             : MFFT_ERR_HIP;
   }
+  const int64_t nxb = (p->stages[1].nsub + p->stages[1].G - 1) / p->stages[1].G;
   for (int64_t h0 = 0; h0 < p->howmany; h0 += p->chunk_h) {
     const int64_t hc = std::min(p->chunk_h, p->howmany - h0);
     StageArgs A = p->stages[0], B = p->stages[1];
     A.pre = pre;
     B.post = post;
+    B.sumsq = sumsq ? sumsq + h0 * nxb : nullptr;
     if (mfft::launch_stage(fwd, A, hc, in + h0 * p->idist, p->d_scratch, p->d_tw, st) !=
         hipSuccess)
       return MFFT_ERR_HIP;
@@ -432,20 +443,29 @@ mfft_plan *mfft_plan_spec(const double *spec, size_t len, unsigned flags) {
 
 int mfft_execute(const mfft_plan *p, const void *in, void *out, void *stream) {
   if (!p) return MFFT_ERR_INVALID;
-  return run(p, (const double2 *)in, (double2 *)out, nullptr, nullptr, (hipStream_t)stream);
+  return run(p, (const double2 *)in, (double2 *)out, nullptr, nullptr, nullptr, (hipStream_t)stream);
 }
 
 int mfft_execute_weighted(const mfft_plan *p, const void *in, void *out, const double *pre,
                           const double *post, void *stream) {
   if (!p) return MFFT_ERR_INVALID;
   if ((pre || post) && (p->istride != 1 || p->ostride != 1)) return MFFT_ERR_UNSUPPORTED;
-  return run(p, (const double2 *)in, (double2 *)out, pre, post, (hipStream_t)stream);
+  return run(p, (const double2 *)in, (double2 *)out, pre, post, nullptr, (hipStream_t)stream);
+}
+
+int64_t mfft_plan_partials(const mfft_plan *p) { return p ? count_partials(p) : -1; }
+
+int mfft_execute_weighted_sumsq(const mfft_plan *p, const void *in, void *out, const double *pre,
+                                const double *post, double *partials, void *stream) {
+  if (!p || !partials) return MFFT_ERR_INVALID;
+  if ((pre || post) && (p->istride != 1 || p->ostride != 1)) return MFFT_ERR_UNSUPPORTED;
+  return run(p, (const double2 *)in, (double2 *)out, pre, post, partials, (hipStream_t)stream);
 }
 
 int mfft_execute_spec(const mfft_plan *p, const double *in, double *out, void *stream) {
   if (!p || !in || !out) return MFFT_ERR_INVALID;
   return run(p, (const double2 *)(in + p->spec_in_off), (double2 *)(out + p->spec_out_off),
-             nullptr, nullptr, (hipStream_t)stream);
+             nullptr, nullptr, nullptr, (hipStream_t)stream);
 }
 
 int mfft_execute_host(const mfft_plan *p, const double *in, double *out) {
@@ -468,7 +488,7 @@ int mfft_execute_host(const mfft_plan *p, const double *in, double *out) {
   if (hipMemcpy(di, in, ni * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(dout, out, no * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess)
     rc = MFFT_ERR_HIP;
-  if (rc == MFFT_SUCCESS) rc = run(p, di, dout, nullptr, nullptr, nullptr);
+  if (rc == MFFT_SUCCESS) rc = run(p, di, dout, nullptr, nullptr, nullptr, nullptr);
   if (rc == MFFT_SUCCESS && (hipDeviceSynchronize() != hipSuccess ||
                              hipMemcpy(out, dout, no * sizeof(double2), hipMemcpyDeviceToHost) !=
                                  hipSuccess))

