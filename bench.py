@@ -150,6 +150,13 @@ def bench_sweep(args, torch, lf):
                     "frac": round(gbs / HBM_PEAK_GBS, 4)})
     bytes_step = 32.0 * total * len(sizes)
     value = bytes_step / t / 1e9
+    # HBM traffic per step from the committed rocprofv3 PMC summary (FETCH_SIZE, WRITE_SIZE,
+    # each scaled by a plain copy kernel), when it describes this sweep
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_pow2_sweep.json")
+    if os.path.exists(tpath) and len(sizes) == 18:
+        with open(tpath) as f:
+            traffic = round(json.load(f)["step_traffic_bytes"])
     t_sum = sum(p["us"] for p in per) / 1e6
     achieved = bytes_step / t_sum / 1e9
     res = {
@@ -162,9 +169,10 @@ def bench_sweep(args, torch, lf):
                                "uniform[-0.5,0.5) re/im; one step = all 18 sizes",
                    "sizes": len(sizes), "elements_per_size": total},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "stage_kernel<true> (all launches of the sweep; algorithmic "
-                               "32N bytes per transform over summed per-size event time)"},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "stage_kernel (all launches of one sweep step: algorithmic 32N "
+                               "bytes per transform over the summed per-size event time; traffic "
+                               "= HBM bytes per step, profiles/traffic_pow2_sweep.json)"},
         "per_size": per,
     }
     if not args.no_cpu_baseline:
