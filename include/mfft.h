@@ -89,6 +89,11 @@ void mfft_destroy_plan(mfft_plan *p);
 int mfft_plan_radices(const mfft_plan *p, int *radices, int max);
 int mfft_plan_stages(const mfft_plan *p, int *passes_per_stage, int max);
 
+/* The schedule mfft_plan_dft_1d would choose for size n, without allocating anything (host
+ * only): the radix sequence (return value: its length, -1 if unsupported) and the number of
+ * passes of the first kernel stage (*split, 0 for one stage). */
+int mfft_plan_schedule(int64_t n, int *radices, int max, int *split);
+
 /* Force a radix sequence (tests and tuning): like mfft_plan_many with ESTIMATE, but the
  * Stockham schedule is `radices` (product n, each radix in {2..16} with prime factors
  * <= 13) and, if stage_split > 0, the first kernel stage runs passes [0, stage_split). */

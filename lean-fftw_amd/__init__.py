@@ -66,6 +66,8 @@ def lib():
                 "mfft_destroy_plan": (None, [vp]),
                 "mfft_plan_radices": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
                 "mfft_plan_stages": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
+                "mfft_plan_schedule": (ctypes.c_int, [i64, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                                      ctypes.POINTER(ctypes.c_int)]),
                 "mfft_plan_ragged": (vp, [i64, ctypes.POINTER(ctypes.c_int64), ctypes.c_int, ctypes.c_uint]),
                 "mfft_execute_ragged": (ctypes.c_int, [vp, vp, vp, vp]),
                 "mfft_ragged_status": (ctypes.c_int, [vp, vp]),
@@ -122,6 +124,16 @@ def pass_twiddles(fwd: bool, N: int, R: int, ns: int):
                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
            "mfft_pass_twiddles")
     return out
+
+
+def schedule(n: int):
+    """(radices, split) the planner chooses for size n (host only, allocates nothing)."""
+    buf = (ctypes.c_int * 64)()
+    split = ctypes.c_int()
+    k = lib().mfft_plan_schedule(n, buf, 64, ctypes.byref(split))
+    if k < 0:
+        raise MfftError(f"unsupported size {n}")
+    return list(buf[:k]), split.value
 
 
 def set_threads(n: int) -> None:

@@ -16,6 +16,18 @@ double tan_2pi(uint64_t num, uint64_t den);
 void cos_sin_2pi_table(uint64_t n, std::vector<double> &cs);  // 2n doubles: c0,s0,c1,s1,...
 
 // ---------------------------------------------------------------- schedule
+// Radix sets of the kernel variants: SMALL holds the radices whose butterflies need few
+// registers, so a stage that uses only them runs at higher occupancy; POW2 is the set of
+// the large-tile variant with 16 elements per thread.
+enum RadixSet { RSET_ALL = 0, RSET_SMALL = 1, RSET_POW2 = 2 };
+__host__ __device__ constexpr bool small_radix(int r) {
+  return r == 2 || r == 3 || r == 4 || r == 5 || r == 6 || r == 8;
+}
+__host__ __device__ constexpr bool in_set(int set, int r) {
+  return set == RSET_ALL ||
+         (set == RSET_SMALL ? small_radix(r) : (r == 2 || r == 4 || r == 8 || r == 16));
+}
+
 constexpr int MAX_PASSES = 12;      // passes per kernel stage
 constexpr int WG_THREADS = 256;     // threads per workgroup of the stage kernels
 constexpr int LDS_ELEMS = 2048;     // complex doubles per LDS buffer, before padding
@@ -23,8 +35,7 @@ constexpr int LDS_ELEMS = 2048;     // complex doubles per LDS buffer, before pa
 // plus one per sub-problem (see mfft_pass.h); the launches request only what their stage
 // needs
 constexpr int kLdsBytes = (LDS_ELEMS + LDS_ELEMS / 16 + LDS_ELEMS / 2) * 16;
-// the large-tile variant: one transform of up to BIG_ELEMS points per workgroup, 16 points
-// per thread, radices 2, 4, 8, 16
+// the large-tile variants: up to BIG_ELEMS points per workgroup, 16 per thread
 constexpr int BIG_ELEMS = 8192;
 constexpr int kLdsBytesBig = (BIG_ELEMS + BIG_ELEMS / 16 + 1) * 16;
 
@@ -49,8 +60,11 @@ struct PassArgs {
 struct StageArgs {
   int L, npass, G;
   int threads;  // workgroup size
-  int big;      // 1: the large-tile variant (one sub-problem of up to BIG_ELEMS per workgroup)
+  int ept;      // elements per thread: 8 (G*L <= LDS_ELEMS, 256 threads) or 16 (large tiles,
+                // G*L <= BIG_ELEMS, up to 512 threads)
   int gfast_in, gfast_out;  // 1: consecutive threads walk g (g adjacent in memory)
+  int stage_io;             // 1: coalesced tile copies HBM -> LDS and back (contiguous small L)
+  uint32_t m_L;             // ceil(2^32 / L), for the tile copies
   PassArgs pass[MAX_PASSES];
   int64_t nsub;
   int64_t in_g, in_e, in_h, out_g, out_e, out_h;

@@ -24,7 +24,7 @@ __global__ __launch_bounds__(MAXT) void stage_kernel(const double2 *in, double2 
   HbmSide hout{out + h * a.out_h + g0 * a.out_g, nullptr, a.out_g, a.out_e,
                a.post ? a.post + g0 * a.post_g : nullptr, a.post_g, a.post_e};
   const double part = run_stage<FWD, RSET, EPT, SQ>(lds, hin, hout, threadIdx.x, blockDim.x, a.G, gcount, a.L,
-                 a.npass, a.pass, a.gfast_in, a.gfast_out, tw, g0, a.kb_u);
+                 a.npass, a.pass, a.gfast_in, a.gfast_out, tw, g0, a.kb_u, a.stage_io, a.m_L);
   if constexpr (SQ) {
     // deterministic per-workgroup partial of sum |out|^2: a fixed reduction tree
     __shared__ double wsum[MAXT / 64];
@@ -41,13 +41,19 @@ __global__ __launch_bounds__(MAXT) void stage_kernel(const double2 *in, double2 
   }
 }
 
-// the kernel instance for (direction, radix set, sum of squares)
-const void *kernel_for(int fwd, int set, int sq) {
+// the kernel instance for (direction, radix set, elements per thread, sum of squares)
+const void *kernel_for(int fwd, int set, int ept, int sq, int big) {
 #define MFFT_K(F, S, E, T)                                                                   \
   (sq ? (const void *)stage_kernel<F, S, E, T, true> : (const void *)stage_kernel<F, S, E, T, false>)
-  if (set == RSET_POW2) return fwd ? MFFT_K(true, RSET_POW2, 16, 512) : MFFT_K(false, RSET_POW2, 16, 512);
-  if (set == RSET_SMALL) return fwd ? MFFT_K(true, RSET_SMALL, 8, 256) : MFFT_K(false, RSET_SMALL, 8, 256);
-  return fwd ? MFFT_K(true, RSET_ALL, 8, 256) : MFFT_K(false, RSET_ALL, 8, 256);
+#define MFFT_DIR(S, E, T) (fwd ? MFFT_K(true, S, E, T) : MFFT_K(false, S, E, T))
+  if (ept == 16) {
+    if (set == RSET_POW2) return MFFT_DIR(RSET_POW2, 16, 512);
+    if (set == RSET_SMALL) return MFFT_DIR(RSET_SMALL, 16, 512);
+    return MFFT_DIR(RSET_ALL, 16, 512);
+  }
+  if (set == RSET_SMALL) return big ? MFFT_DIR(RSET_SMALL, 8, 1024) : MFFT_DIR(RSET_SMALL, 8, 256);
+  return MFFT_DIR(RSET_ALL, 8, 256);
+#undef MFFT_DIR
 #undef MFFT_K
 }
 
@@ -55,19 +61,25 @@ hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const doub
                         double2 *out, const double2 *tw, hipStream_t stream) {
   if (a.nsub <= 0 || ntrans <= 0) return hipSuccess;
   const int64_t nx = (a.nsub + a.G - 1) / a.G;
-  const size_t lds = (size_t)stage_lds_elems(a.G, a.L, a.npass) * sizeof(double2);
+  const size_t lds = (size_t)stage_lds_elems(a.G, a.L, a.npass, a.stage_io) * sizeof(double2);
   const dim3 grid((unsigned)nx, (unsigned)ntrans);
   // variants: <direction, radix set, elements per thread, max threads, sum of squares>
-  bool small = true;
-  for (int p = 0; p < a.npass; ++p) small = small && small_radix(a.pass[p].R);
-  const int set = a.big ? RSET_POW2 : (small ? RSET_SMALL : RSET_ALL);
-  const void *k = kernel_for(fwd, set, a.sumsq != nullptr);
+  bool small = true, pow2 = true;
+  for (int p = 0; p < a.npass; ++p) {
+    const int r = a.pass[p].R;
+    small = small && small_radix(r);
+    pow2 = pow2 && (r == 2 || r == 4 || r == 8 || r == 16);
+  }
+  const int set = (a.ept == 16 && pow2) ? RSET_POW2 : (small ? RSET_SMALL : RSET_ALL);
+  const void *k = kernel_for(fwd, set, a.ept, a.sumsq != nullptr, a.threads > 256);
   static const bool attr = [] {
     for (int f = 0; f < 2; ++f)
       for (int st = 0; st < 3; ++st)
-        for (int q = 0; q < 2; ++q)
-          (void)hipFuncSetAttribute(kernel_for(f, st, q), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    kLdsBytesBig);
+        for (int e = 8; e <= 16; e += 8)
+          for (int q = 0; q < 2; ++q)
+            for (int b = 0; b < 2; ++b)
+              (void)hipFuncSetAttribute(kernel_for(f, st, e, q, b),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytesBig);
     return true;
   }();
   (void)attr;

@@ -20,9 +20,9 @@ __device__ __forceinline__ int lpad(int e) { return e + (e >> 4); }
 __host__ __device__ constexpr int lds_len(int L) { return L + (L >> 4) + 1; }
 
 // LDS of a stage: none for one pass (HBM to HBM), one buffer of G padded sub-problems
-// otherwise (the intermediate passes run in place).
-__host__ __device__ constexpr int stage_lds_elems(int G, int L, int npass) {
-  return npass <= 1 ? 0 : G * lds_len(L);
+// otherwise (the intermediate passes run in place), and always when the tile is staged.
+__host__ __device__ constexpr int stage_lds_elems(int G, int L, int npass, int stage_io = 0) {
+  return (npass <= 1 && !stage_io) ? 0 : G * lds_len(L);
 }
 
 // x * w for a twiddle input, exactly as the reference's general product mulGen(w, x)
@@ -32,17 +32,6 @@ __host__ __device__ constexpr int stage_lds_elems(int G, int L, int npass) {
 // generator sees through the product, so both DAGs round identically.
 __device__ __forceinline__ double2 twmul(double2 w, double2 x) {
   return make_double2(fma(w.x, x.x, -(w.y * x.y)), fma(w.x, x.y, w.y * x.x));
-}
-
-// Radix sets of the kernel variants: SMALL holds the radices whose butterflies need few
-// registers, so a stage that uses only them runs at higher occupancy; POW2 is the set of
-// the large-tile variant.
-enum RadixSet { RSET_ALL = 0, RSET_SMALL = 1, RSET_POW2 = 2 };
-__host__ __device__ constexpr bool small_radix(int r) {
-  return r == 2 || r == 3 || r == 4 || r == 5 || r == 6 || r == 8;
-}
-__host__ __device__ constexpr bool in_set(int set, int r) {
-  return set == RSET_ALL || (set == RSET_SMALL ? small_radix(r) : (r == 2 || r == 4 || r == 8 || r == 16));
 }
 
 // q = n / d for n * d < 2^32, with m = ceil(2^32 / d) (m = 0 encodes d = 1)
@@ -165,19 +154,76 @@ __device__ __forceinline__ double pass_run(double2 *buf, const HbmSide &hin, con
   return sumsq;
 }
 
-// All passes of a stage: HBM -> (LDS ->)* HBM. Twiddle index of pass p for local index k
-// of sub-problem g: (kb0 + g) * kb_u + k * kmul.
+// Coalesced copy of a tile of contiguous sub-problems between HBM and LDS: consecutive lanes
+// take consecutive elements f = g * L + i (sub-problem g at g * gstride, gstride == L).
+template <int EPT>
+__device__ __forceinline__ void tile_to_lds(double2 *buf, const HbmSide &h, int tid, int nthreads,
+                                            int gcount, int L, uint32_t m_L) {
+  const int total = gcount * L, ll = lds_len(L);
+  double2 v[EPT];
+#pragma unroll
+  for (int u = 0; u < EPT; ++u) {
+    const int f = tid + u * nthreads;
+    if (f < total) {
+      v[u] = h.base[f];
+      if (h.wt) {
+        const int g = (int)fdiv((uint32_t)f, m_L);
+        const double w = h.wt[(int64_t)g * h.wg + (int64_t)(f - g * L) * h.wstride];
+        v[u].x = v[u].x * w;
+        v[u].y = v[u].y * w;
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < EPT; ++u) {
+    const int f = tid + u * nthreads;
+    if (f < total) {
+      const int g = (int)fdiv((uint32_t)f, m_L);
+      buf[g * ll + lpad(f - g * L)] = v[u];
+    }
+  }
+  __syncthreads();
+}
+
+template <int EPT, bool SQ>
+__device__ __forceinline__ double tile_from_lds(const double2 *buf, const HbmSide &h, int tid,
+                                                int nthreads, int gcount, int L, uint32_t m_L) {
+  const int total = gcount * L, ll = lds_len(L);
+  double sumsq = 0.0;
+#pragma unroll
+  for (int u = 0; u < EPT; ++u) {
+    const int f = tid + u * nthreads;
+    if (f < total) {
+      const int g = (int)fdiv((uint32_t)f, m_L);
+      double2 v = buf[g * ll + lpad(f - g * L)];
+      if (h.wt) {
+        const double w = h.wt[(int64_t)g * h.wg + (int64_t)(f - g * L) * h.wstride];
+        v.x = v.x * w;
+        v.y = v.y * w;
+      }
+      h.base[f] = v;
+      if constexpr (SQ) sumsq = sumsq + (v.x * v.x + v.y * v.y);
+    }
+  }
+  return sumsq;
+}
+
+// All passes of a stage: HBM -> (LDS ->)* HBM, or with `stage_io` the tile is first copied
+// to LDS and finally back to HBM by coalesced tile copies (for contiguous small
+// sub-problems, whose butterfly reads would otherwise leave lanes far apart). Twiddle index
+// of pass p for local index k of sub-problem g: (kb0 + g) * kb_u + k * kmul.
 template <bool FWD, int RSET, int EPT, bool SQ = false>
 __device__ __forceinline__ double run_stage(double2 *buf, const HbmSide &hin, const HbmSide &hout,
                                           int tid, int nthreads, int G, int gcount, int L,
-                                          int npass,
-                                          const PassArgs *pas, int gfast_in, int gfast_out,
-                                          const double2 *__restrict__ tw, int64_t kb0,
-                                          int64_t kb_u) {
+                                          int npass, const PassArgs *pas, int gfast_in,
+                                          int gfast_out, const double2 *__restrict__ tw,
+                                          int64_t kb0, int64_t kb_u, int stage_io = 0,
+                                          uint32_t m_L = 0) {
   double sumsq = 0.0;
+  if (stage_io) tile_to_lds<EPT>(buf, hin, tid, nthreads, gcount, L, m_L);
   for (int p = 0; p < npass; ++p) {
     const PassArgs pa = pas[p];
-    const bool first = p == 0, last = p == npass - 1;
+    const bool first = p == 0 && !stage_io, last = p == npass - 1 && !stage_io;
     const bool gfast = first ? gfast_in : (last ? gfast_out : false);
 #define MFFT_CASE(RR)                                                                            \
   case RR:                                                                                       \
@@ -220,6 +266,7 @@ __device__ __forceinline__ double run_stage(double2 *buf, const HbmSide &hin, co
     }
 #undef MFFT_CASE
   }
+  if (stage_io) sumsq = tile_from_lds<EPT, SQ>(buf, hout, tid, nthreads, gcount, L, m_L);
   return sumsq;
 }
 

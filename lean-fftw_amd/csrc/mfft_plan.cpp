@@ -68,10 +68,28 @@ int64_t chunk_bytes() {
   return v;
 }
 
+// Largest transform length whose single-stage tile is staged through LDS (coalesced copies).
+int64_t stage_io_max() {
+  static const int64_t v = [] {
+    const char *e = std::getenv("MFFT_STAGE_IO_MAX");
+    return e ? (int64_t)std::atoll(e) : (int64_t)16;
+  }();
+  return v;
+}
+
 bool smooth13(int64_t n) {
   for (int p : {2, 3, 5, 7, 11, 13})
     while (n % p == 0) n /= p;
   return n == 1;
+}
+
+// Largest radix the planner uses by itself (MFFT_MAX_RADIX, default 16).
+int max_radix() {
+  static const int v = [] {
+    const char *e = std::getenv("MFFT_MAX_RADIX");
+    return e ? std::max(2, std::min(16, std::atoi(e))) : 16;
+  }();
+  return v;
 }
 
 bool radix_ok(int r) { return r >= 2 && r <= 16 && smooth13(r); }
@@ -80,10 +98,10 @@ bool radix_ok(int r) { return r >= 2 && r <= 16 && smooth13(r); }
 // fixed charge for the LDS round trip of the pass.
 double pass_cost(int r) { return (double)mfft::kBflyOps[r] / r + 6.0; }
 
-// Radix sequence for an LDS stage of length L: the fewest passes (every pass is one
-// LDS round trip), then the least arithmetic; radices in descending order.
-std::vector<int> choose_radices(int64_t L) {
-  // memoised DP over divisors
+// Best radix sequence of length L over the radices accepted by `ok` (<= max_radix()): the
+// fewest passes (every pass is one LDS round trip), then the least arithmetic. Returns
+// false if L has no such factorisation. `cost` receives passes * 1000 + arithmetic.
+bool best_radices(int64_t L, bool (*ok)(int), std::vector<int> &out, double &cost) {
   std::vector<std::pair<double, std::vector<int>>> best;
   std::vector<int64_t> divs;
   for (int64_t d = 1; d <= L; ++d)
@@ -92,11 +110,10 @@ std::vector<int> choose_radices(int64_t L) {
   best[0] = {0.0, {}};
   for (size_t i = 1; i < divs.size(); ++i) {
     const int64_t d = divs[i];
-    for (int r = 16; r >= 2; --r) {
-      if (!radix_ok(r) || d % r) continue;
+    for (int r = max_radix(); r >= 2; --r) {
+      if (!radix_ok(r) || !ok(r) || d % r) continue;
       const size_t j = std::lower_bound(divs.begin(), divs.end(), d / r) - divs.begin();
       if (best[j].first >= 1e299) continue;
-      // lexicographic (passes, cost): a pass costs 1000 + arithmetic
       const double c = best[j].first + 1000.0 + pass_cost(r);
       if (c < best[i].first) {
         best[i].first = c;
@@ -105,9 +122,32 @@ std::vector<int> choose_radices(int64_t L) {
       }
     }
   }
-  std::vector<int> rs = best.back().second;
-  std::sort(rs.rbegin(), rs.rend());
-  return rs;
+  if (best.back().first >= 1e299) return false;
+  out = best.back().second;
+  std::sort(out.rbegin(), out.rend());
+  cost = best.back().first;
+  return true;
+}
+
+bool any_radix(int) { return true; }
+bool small_radix_ok(int r) { return mfft::small_radix(r); }
+
+// Radix sequence for an LDS stage of length L. A stage whose radices are all small
+// (2, 3, 4, 5, 6, 8) runs in the small-radix kernel variant at higher occupancy (about 88
+// VGPRs against 134), which measured faster than saving a pass with radix 16 (512 points:
+// 5.2 TB/s as 8x8x8 against 3.9 as 16x16x2); so the unrestricted sequence is charged 600
+// per pass for its register cost when it uses a larger radix.
+std::vector<int> choose_radices(int64_t L) {
+  std::vector<int> all, small;
+  double c_all = 1e300, c_small = 1e300;
+  best_radices(L, any_radix, all, c_all);
+  if (best_radices(L, small_radix_ok, small, c_small)) {
+    bool all_is_small = true;
+    for (int r : all) all_is_small = all_is_small && mfft::small_radix(r);
+    if (!all_is_small) c_all += 600.0 * all.size();
+    if (c_small <= c_all) return small;
+  }
+  return all;
 }
 
 // Split N into P * Q for a two-stage plan: both within the LDS, as balanced as possible.
@@ -165,6 +205,48 @@ hipError_t build_twiddles(mfft_plan *p) {
   return hipSuccess;
 }
 
+bool all_small(const std::vector<int> &rs) {
+  for (int r : rs)
+    if (!mfft::small_radix(r)) return false;
+  return true;
+}
+
+bool all_pow2(const std::vector<int> &rs) {
+  for (int r : rs)
+    if (r != 2 && r != 4 && r != 8 && r != 16) return false;
+  return true;
+}
+
+// Workgroup shape of a stage whose G, L and radices `rs` are set: 256 threads with 8
+// elements each; or, for a large tile (G * L > LDS_ELEMS), 8 elements per thread and up to
+// 1024 threads when every radix is small, else 16 elements per thread and up to 512 threads.
+// Thread counts are multiples of 64.
+void set_tile(StageArgs &a, bool big, const std::vector<int> &rs) {
+  if (!big) {
+    a.ept = 8;
+    a.threads = mfft::WG_THREADS;
+    return;
+  }
+  a.ept = all_small(rs) ? 8 : 16;
+  const int64_t elems = (int64_t)a.G * a.L;
+  const int64_t per = a.ept * 64;
+  a.threads = (int)std::min<int64_t>(a.ept == 8 ? 1024 : 512, ((elems + per - 1) / per) * 64);
+}
+
+// A stage over sub-problems adjacent in memory (sub-problem g at g, element e at e*stride):
+// G = 8 sub-problems (128-byte rows) when L allows, in a large tile if L > LDS_ELEMS / 8 and
+// the stage's radices keep a large tile's registers low (power-of-two or small radices).
+void strided_tile(StageArgs &a, int64_t L, int64_t U, bool allow_big, const std::vector<int> &rs) {
+  int64_t G = std::max<int64_t>(1, mfft::LDS_ELEMS / L);
+  bool big = false;
+  if (G < 8 && allow_big && (all_small(rs) || all_pow2(rs))) {
+    G = std::max<int64_t>(1, std::min<int64_t>(8, mfft::BIG_ELEMS / L));
+    big = G * L > mfft::LDS_ELEMS;
+  }
+  a.G = (int)std::min<int64_t>(G, U);
+  set_tile(a, big && a.G * L > mfft::LDS_ELEMS, rs);
+}
+
 void fill_pass(mfft::PassArgs &pa, int R, int64_t nsl, int64_t twns, int64_t kmul, int L,
                int G) {
   pa.R = R;
@@ -187,16 +269,14 @@ bool build_stages(mfft_plan *p, int split) {
   std::vector<int64_t> ns(s + 1, 1);
   for (int t = 0; t < s; ++t) ns[t + 1] = ns[t] * rs[t];
   if (split <= 0 || split >= s) {
-    bool pow2 = true;
-    for (int r : rs) pow2 = pow2 && (r == 2 || r == 4 || r == 8 || r == 16);
-    const bool big = N > mfft::LDS_ELEMS && N <= mfft::BIG_ELEMS && pow2 && p->allow_big;
+    const bool big = N > mfft::LDS_ELEMS && p->allow_big &&
+                     N <= ((N & (N - 1)) == 0 ? mfft::BIG_ELEMS : mfft::BIG_ELEMS / 2);
     if ((N > mfft::LDS_ELEMS && !big) || s > mfft::MAX_PASSES) return false;
     StageArgs a{};
     a.L = (int)N;
     a.npass = s;
-    a.big = big ? 1 : 0;
-    a.threads = big ? (int)(N / 16) : mfft::WG_THREADS;
     a.G = big ? 1 : (int)std::max<int64_t>(1, std::min<int64_t>(mfft::LDS_ELEMS / N, p->howmany));
+    set_tile(a, big, rs);
     for (int t = 0; t < s; ++t) fill_pass(a.pass[t], rs[t], ns[t], ns[t], 1, a.L, a.G);
     a.nsub = p->howmany;
     a.in_g = p->idist;
@@ -212,6 +292,12 @@ bool build_stages(mfft_plan *p, int split) {
     a.pre_e = 1;
     a.post_g = 0;
     a.post_e = 1;
+    // small contiguous transforms: stage the tile through LDS with coalesced copies
+    a.m_L = mfft::magic32(N);
+    a.stage_io = (N <= stage_io_max() && a.ept == 8 && p->istride == 1 && p->ostride == 1 &&
+                  p->idist == N && p->odist == N)
+                     ? 1
+                     : 0;
     p->stages.push_back(a);
     p->stage_passes.push_back(s);
     p->chunk_h = p->howmany;
@@ -222,10 +308,9 @@ bool build_stages(mfft_plan *p, int split) {
       s - split > mfft::MAX_PASSES)
     return false;
   StageArgs A{}, B{};
-  A.threads = B.threads = mfft::WG_THREADS;
   A.L = (int)P;
   A.npass = split;
-  A.G = (int)std::max<int64_t>(1, std::min<int64_t>(mfft::LDS_ELEMS / P, Q));
+  strided_tile(A, P, Q, p->allow_big, std::vector<int>(rs.begin(), rs.begin() + split));
   for (int t = 0; t < split; ++t) fill_pass(A.pass[t], rs[t], ns[t], ns[t], 1, A.L, A.G);
   A.nsub = Q;
   A.in_g = p->istride;  // column c starts at element c
@@ -242,7 +327,7 @@ bool build_stages(mfft_plan *p, int split) {
 
   B.L = (int)Q;
   B.npass = s - split;
-  B.G = (int)std::max<int64_t>(1, std::min<int64_t>(mfft::LDS_ELEMS / Q, P));
+  strided_tile(B, Q, P, p->allow_big, std::vector<int>(rs.begin() + split, rs.end()));
   for (int t = split; t < s; ++t)
     fill_pass(B.pass[t - split], rs[t], ns[t] / P, ns[t], P, B.L, B.G);
   B.nsub = P;  // residue classes k1 + P i
@@ -342,7 +427,8 @@ mfft_plan *plan_many_internal(int64_t n, int sign, int64_t howmany, int64_t istr
   p->allow_big = allow_big;
   const bool pow2 = (n & (n - 1)) == 0;
   int split = 0;
-  if (n <= mfft::LDS_ELEMS || (allow_big && pow2 && n <= mfft::BIG_ELEMS)) {
+  if (n <= mfft::LDS_ELEMS ||
+      (allow_big && n <= (pow2 ? mfft::BIG_ELEMS : mfft::BIG_ELEMS / 2))) {
     p->radices = choose_radices(n);
   } else {
     int64_t P;
@@ -361,7 +447,7 @@ mfft_plan *plan_many_internal(int64_t n, int sign, int64_t howmany, int64_t istr
 int mfft::export_class(const mfft_plan *p, mfft::RClass *c, std::vector<double2> *tw) {
   if (!p || p->howmany != 1 || p->stages.empty() || p->stages.size() > 2) return 1;
   for (const auto &st : p->stages)
-    if (st.big) return 1;  // the ragged kernel runs the 256-thread layout only
+    if (st.ept != 8) return 1;  // the ragged kernel runs the 256-thread layout only
   const int64_t base = (int64_t)tw->size();
   *c = mfft::RClass{};
   c->N = (int)p->n;
@@ -410,6 +496,26 @@ mfft_plan *mfft_plan_with_radices(int64_t n, int sign, int64_t howmany, const in
 mfft_plan *mfft_plan_many(int64_t n, int sign, int64_t howmany, int64_t istride, int64_t idist,
                           int64_t ostride, int64_t odist, unsigned flags) {
   return plan_many_internal(n, sign, howmany, istride, idist, ostride, odist, flags, true);
+}
+
+int mfft_plan_schedule(int64_t n, int *radices, int max, int *split) {
+  if (!valid_problem(n, -1, 1)) return -1;
+  std::vector<int> rs;
+  int sp = 0;
+  const bool pow2 = (n & (n - 1)) == 0;
+  if (n <= mfft::LDS_ELEMS || n <= (pow2 ? mfft::BIG_ELEMS : mfft::BIG_ELEMS / 2)) {
+    rs = choose_radices(n);
+  } else {
+    int64_t P;
+    if (!choose_split(n, P)) return -1;
+    std::vector<int> a = choose_radices(P), b = choose_radices(n / P);
+    rs = a;
+    rs.insert(rs.end(), b.begin(), b.end());
+    sp = (int)a.size();
+  }
+  for (int i = 0; i < (int)rs.size() && i < max; ++i) radices[i] = rs[i];
+  if (split) *split = sp;
+  return (int)rs.size();
 }
 
 mfft_plan *mfft_plan_dft_1d(int64_t n, int sign, int64_t howmany, unsigned flags) {

@@ -126,3 +126,24 @@ def test_missing_library_fails_loudly(lf, tmp_path, monkeypatch):
     monkeypatch.setattr(lf, "_lib", None)
     with pytest.raises(lf.MfftError):
         lf.lib()
+
+
+def test_planner_schedules_are_valid(lf):
+    """Every schedule multiplies to N, uses radices 2..16 with prime factors <= 13, and its
+    kernel stages fit the LDS tiles (no GPU needed: the planner query allocates nothing)."""
+    import math
+
+    w = lf.workloads
+    sizes = sorted(set([2 ** k for k in range(1, 19)] + w.smooth_sizes(2, 3000)[::7] +
+                       [10368, 27000, 75600, 165375, 65520, 1 << 20]))
+    for n in sizes:
+        rs, split = lf.schedule(n)
+        assert math.prod(rs) == n, (n, rs)
+        assert all(2 <= r <= 16 and w.smooth13(r) for r in rs), (n, rs)
+        if split:
+            P = math.prod(rs[:split])
+            assert P <= 2048 and n // P <= 2048, (n, rs, split)
+        else:
+            assert n <= 8192, (n, rs)
+    with pytest.raises(lf.MfftError):
+        lf.schedule(17)
