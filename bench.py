@@ -121,7 +121,8 @@ def bench_sweep(args, torch, lf):
     total = 1 << 24
     x = uniform_complex(torch, (total,), 20261203)
     y = torch.empty_like(x)
-    plans = {N: lf.Plan(N, lf.FORWARD, total // N) for N in sizes}
+    flags = lf.MEASURE if args.measure else lf.ESTIMATE
+    plans = {N: lf.Plan(N, lf.FORWARD, total // N, flags=flags) for N in sizes}
     views = {N: (x.view(total // N, N), y.view(total // N, N)) for N in sizes}
 
     def step():
@@ -167,7 +168,8 @@ def bench_sweep(args, torch, lf):
         "config": {"workload": "batched power-of-two sweep: complex FP64 forward, out of place, "
                                "N=2^1..2^18, batch 2^24/N per size (256 MB in + 256 MB out), "
                                "uniform[-0.5,0.5) re/im; one step = all 18 sizes",
-                   "sizes": len(sizes), "elements_per_size": total},
+                   "sizes": len(sizes), "elements_per_size": total,
+                   "planner": "measure" if args.measure else "estimate"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "stage_kernel (all launches of one sweep step: algorithmic 32N "
@@ -413,6 +415,8 @@ def main():
     ap.add_argument("--sizes", type=int, nargs="*", help="restrict the sweep to these sizes")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--measure", action="store_true",
+                    help="plan with MFFT_MEASURE (time candidate schedules before the run)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (a batch shards "

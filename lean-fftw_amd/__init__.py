@@ -140,6 +140,12 @@ def set_threads(n: int) -> None:
     _check(lib().mfft_set_threads(n), "mfft_set_threads")
 
 
+def set_wisdom_path(path) -> None:
+    """JSON-lines file where MEASURE plans look up and append their timed schedules.
+    None or "" falls back to the MFFT_WISDOM environment variable (no cache when unset)."""
+    _check(lib().mfft_set_wisdom_path((path or "").encode()), "mfft_set_wisdom_path")
+
+
 # ------------------------------------------------------------------------------ plans
 
 

@@ -101,7 +101,9 @@ double pass_cost(int r) { return (double)mfft::kBflyOps[r] / r + 6.0; }
 // Best radix sequence of length L over the radices accepted by `ok` (<= max_radix()): the
 // fewest passes (every pass is one LDS round trip), then the least arithmetic. Returns
 // false if L has no such factorisation. `cost` receives passes * 1000 + arithmetic.
-bool best_radices(int64_t L, bool (*ok)(int), std::vector<int> &out, double &cost) {
+bool best_radices(int64_t L, bool (*ok)(int), std::vector<int> &out, double &cost,
+                  int cap = 0) {
+  if (cap <= 0) cap = max_radix();
   std::vector<std::pair<double, std::vector<int>>> best;
   std::vector<int64_t> divs;
   for (int64_t d = 1; d <= L; ++d)
@@ -110,7 +112,7 @@ bool best_radices(int64_t L, bool (*ok)(int), std::vector<int> &out, double &cos
   best[0] = {0.0, {}};
   for (size_t i = 1; i < divs.size(); ++i) {
     const int64_t d = divs[i];
-    for (int r = max_radix(); r >= 2; --r) {
+    for (int r = cap; r >= 2; --r) {
       if (!radix_ok(r) || !ok(r) || d % r) continue;
       const size_t j = std::lower_bound(divs.begin(), divs.end(), d / r) - divs.begin();
       if (best[j].first >= 1e299) continue;
@@ -409,12 +411,214 @@ int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
   return MFFT_SUCCESS;
 }
 
+// ------------------------------------------------------------------ autotuner (MFFT_MEASURE)
+//
+// The model of the reference's measuring planner (LeanFFTW/Asm/Planner.lean:224-286:
+// enumerate radix sequences, order them by operation count, time them, keep the fastest):
+// candidate schedules are the cost-model sequences for radix caps 4, 8 and 16, restricted to
+// the small radices or not, in descending and ascending order, and for two-stage sizes the
+// three most balanced splits; each is built, run once, then timed (min of 3) with HIP events
+// on scratch buffers of up to 64 MB of the plan's own layout. The winner is cached per
+// (device, size, direction, batch class, layout) in a JSON-lines wisdom file.
+
+struct Cand {
+  std::vector<int> rs;
+  int split;
+};
+
+std::vector<std::vector<int>> seq_candidates(int64_t L) {
+  std::vector<std::vector<int>> out;
+  auto add = [&](std::vector<int> v) {
+    if (v.empty()) return;
+    if (std::find(out.begin(), out.end(), v) == out.end()) out.push_back(v);
+    std::reverse(v.begin(), v.end());
+    if (std::find(out.begin(), out.end(), v) == out.end()) out.push_back(v);
+  };
+  add(choose_radices(L));
+  for (int cap : {4, 8, 16}) {
+    std::vector<int> v;
+    double c;
+    if (best_radices(L, any_radix, v, c, cap)) add(v);
+    if (best_radices(L, small_radix_ok, v, c, cap)) add(v);
+  }
+  return out;
+}
+
+std::vector<Cand> schedule_candidates(int64_t n, bool allow_big) {
+  std::vector<Cand> out;
+  const bool pow2 = (n & (n - 1)) == 0;
+  if (n <= mfft::LDS_ELEMS || (allow_big && n <= (pow2 ? mfft::BIG_ELEMS : mfft::BIG_ELEMS / 2)))
+    for (auto &v : seq_candidates(n)) out.push_back({v, 0});
+  if (n > mfft::LDS_ELEMS) {
+    std::vector<std::pair<double, int64_t>> splits;
+    for (int64_t P = 2; P <= mfft::LDS_ELEMS && P < n; ++P)
+      if (n % P == 0 && n / P <= mfft::LDS_ELEMS)
+        splits.push_back({std::fabs(std::log((double)P) - std::log((double)(n / P))), P});
+    std::sort(splits.begin(), splits.end());
+    for (size_t i = 0; i < splits.size() && i < 3; ++i) {
+      const int64_t P = splits[i].second;
+      auto A = seq_candidates(P), B = seq_candidates(n / P);
+      for (size_t a = 0; a < A.size() && a < 3; ++a)
+        for (size_t b = 0; b < B.size() && b < 3; ++b) {
+          Cand c{A[a], (int)A[a].size()};
+          c.rs.insert(c.rs.end(), B[b].begin(), B[b].end());
+          out.push_back(c);
+        }
+    }
+  }
+  return out;
+}
+
+std::string device_key() {
+  int dev = 0, ncu = 0;
+  (void)hipGetDevice(&dev);
+  hipDeviceProp_t prop{};
+  (void)hipGetDeviceProperties(&prop, dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  std::string arch = prop.gcnArchName;
+  const size_t colon = arch.find(':');
+  if (colon != std::string::npos) arch = arch.substr(0, colon);
+  return arch + ":" + std::to_string(ncu);
+}
+
+std::string wisdom_file() {
+  std::lock_guard<std::mutex> lk(g_wisdom_mu);
+  if (!g_wisdom_path.empty()) return g_wisdom_path;
+  const char *e = std::getenv("MFFT_WISDOM");
+  return e ? std::string(e) : std::string();
+}
+
+std::string wisdom_key(const mfft_plan *p) {
+  int hb = 0;
+  while ((int64_t(1) << (hb + 1)) <= p->howmany) ++hb;
+  const bool contig = p->istride == 1 && p->ostride == 1 && p->idist == p->n && p->odist == p->n;
+  return device_key() + "|" + std::to_string(p->n) + "|" + std::to_string(p->sign) + "|" +
+         std::to_string(hb) + "|" + (contig ? "c" : "s") + "|" + (p->allow_big ? "b" : "n");
+}
+
+// wisdom lines: {"key": "<key>", "rs": [r0, r1, ...], "split": s, "us": t}
+bool wisdom_lookup(const std::string &key, int64_t key_n, Cand &c) {
+  const std::string path = wisdom_file();
+  if (path.empty()) return false;
+  FILE *f = std::fopen(path.c_str(), "r");
+  if (!f) return false;
+  char line[1024];
+  bool found = false;
+  const std::string want = "\"key\": \"" + key + "\"";
+  while (std::fgets(line, sizeof line, f)) {
+    const std::string sl(line);
+    if (sl.find(want) == std::string::npos) continue;
+    const size_t a = sl.find("\"rs\": ["), b = sl.find(']', a), sp = sl.find("\"split\": ");
+    if (a == std::string::npos || b == std::string::npos || sp == std::string::npos) continue;
+    Cand t{{}, std::atoi(sl.c_str() + sp + 9)};
+    const std::string body = sl.substr(a + 7, b - a - 7);
+    size_t pos = 0;
+    while (pos < body.size()) {
+      const size_t nx = body.find(',', pos);
+      const std::string tok = body.substr(pos, nx == std::string::npos ? std::string::npos : nx - pos);
+      if (!tok.empty()) t.rs.push_back(std::atoi(tok.c_str()));
+      if (nx == std::string::npos) break;
+      pos = nx + 1;
+    }
+    int64_t prod = 1;
+    bool ok = !t.rs.empty() && t.split >= 0 && t.split < (int)t.rs.size();
+    for (int r : t.rs) {
+      ok = ok && radix_ok(r);
+      prod *= ok ? r : 1;
+    }
+    if (!ok || prod != key_n) continue;
+    c = t;  // the last valid matching line wins
+    found = true;
+  }
+  std::fclose(f);
+  return found;
+}
+
+void wisdom_store(const std::string &key, const Cand &c, double us) {
+  const std::string path = wisdom_file();
+  if (path.empty()) return;
+  std::lock_guard<std::mutex> lk(g_wisdom_mu);
+  FILE *f = std::fopen(path.c_str(), "a");
+  if (!f) return;
+  std::string rs;
+  for (size_t i = 0; i < c.rs.size(); ++i) rs += (i ? ", " : "") + std::to_string(c.rs[i]);
+  std::fprintf(f, "{\"key\": \"%s\", \"rs\": [%s], \"split\": %d, \"us\": %.3f}\n",
+               key.c_str(), rs.c_str(), c.split, us);
+  std::fclose(f);
+}
+
+mfft_plan *clone_shape(const mfft_plan *p, int64_t howmany) {
+  auto *q = new mfft_plan;
+  q->n = p->n;
+  q->sign = p->sign;
+  q->howmany = howmany;
+  q->istride = p->istride;
+  q->idist = p->idist;
+  q->ostride = p->ostride;
+  q->odist = p->odist;
+  q->allow_big = p->allow_big;
+  return q;
+}
+
+// Time the candidates on scratch buffers; returns false (and leaves `best` alone) when the
+// layout's span is too large to tune on scratch, or nothing could be timed.
+bool measure(const mfft_plan *shape, Cand &best) {
+  const int64_t n = shape->n;
+  const int64_t hm = std::max<int64_t>(1, std::min<int64_t>(shape->howmany, (int64_t(1) << 22) / n));
+  auto span = [&](int64_t st, int64_t d) { return std::llabs(st) * (n - 1) + std::llabs(d) * (hm - 1) + 1; };
+  const int64_t si = span(shape->istride, shape->idist), so = span(shape->ostride, shape->odist);
+  if (si > (int64_t(1) << 25) || so > (int64_t(1) << 25)) return false;
+  if (shape->istride < 0 || shape->ostride < 0 || shape->idist < 0 || shape->odist < 0) return false;
+  double2 *din = nullptr, *dout = nullptr;
+  if (hipMalloc(&din, si * sizeof(double2)) != hipSuccess) return false;
+  if (hipMalloc(&dout, so * sizeof(double2)) != hipSuccess) {
+    (void)hipFree(din);
+    return false;
+  }
+  (void)hipMemset(din, 0, si * sizeof(double2));
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  (void)hipStreamCreate(&st);
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  double best_us = 1e300;
+  for (const Cand &c : schedule_candidates(n, shape->allow_big)) {
+    mfft_plan *q = clone_shape(shape, hm);
+    q->radices = c.rs;
+    q = finish(q, c.split);
+    if (!q) continue;
+    double t_min = 1e300;
+    if (run(q, din, dout, nullptr, nullptr, nullptr, st) == MFFT_SUCCESS) {
+      for (int rep = 0; rep < 3; ++rep) {
+        (void)hipEventRecord(e0, st);
+        (void)run(q, din, dout, nullptr, nullptr, nullptr, st);
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) t_min = std::min(t_min, (double)ms * 1e3);
+      }
+    }
+    mfft_destroy_plan(q);
+    if (t_min < best_us) {
+      best_us = t_min;
+      best = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(st);
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  if (best_us >= 1e299) return false;
+  wisdom_store(wisdom_key(shape), best, best_us);
+  return true;
+}
+
 }  // namespace
 
 mfft_plan *plan_many_internal(int64_t n, int sign, int64_t howmany, int64_t istride,
                               int64_t idist, int64_t ostride, int64_t odist, unsigned flags,
                               bool allow_big) {
-  (void)flags;
   if (!valid_problem(n, sign, howmany) || istride == 0 || ostride == 0) return nullptr;
   auto *p = new mfft_plan;
   p->n = n;
@@ -427,6 +631,19 @@ mfft_plan *plan_many_internal(int64_t n, int sign, int64_t howmany, int64_t istr
   p->allow_big = allow_big;
   const bool pow2 = (n & (n - 1)) == 0;
   int split = 0;
+  if (flags & MFFT_MEASURE) {
+    // A stale or hand-edited wisdom line that does not describe a valid schedule for this
+    // size is ignored; the default schedule is used when nothing can be measured.
+    Cand c;
+    if (wisdom_lookup(wisdom_key(p), n, c) || measure(p, c)) {
+      mfft_plan *q = clone_shape(p, howmany);
+      q->radices = c.rs;
+      if ((q = finish(q, c.split))) {
+        delete p;
+        return q;
+      }
+    }
+  }
   if (n <= mfft::LDS_ELEMS ||
       (allow_big && n <= (pow2 ? mfft::BIG_ELEMS : mfft::BIG_ELEMS / 2))) {
     p->radices = choose_radices(n);

@@ -1,0 +1,94 @@
+"""The measuring planner (MFFT_MEASURE): whatever schedule it times fastest, the result is
+bit-identical to the oracle's refTransform with that radix sequence; the choice is cached in
+the JSON-lines wisdom file and read back; lines that do not describe a valid schedule for
+the size are ignored."""
+
+import json
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = pytest.mark.gpu
+
+if not gpu_available():  # pragma: no cover
+    pytest.skip("needs a GPU", allow_module_level=True)
+
+import torch  # noqa: E402
+
+DEV = "cuda:0"
+
+
+def bits_equal(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint64), np.ascontiguousarray(b).view(np.uint64))
+
+
+def run_plan(p, x):
+    xd = torch.from_numpy(x).to(DEV)
+    y = torch.empty_like(xd)
+    p.call(xd, y)
+    torch.cuda.synchronize()
+    return y.cpu().numpy()
+
+
+@pytest.mark.parametrize("n,sign", [(1024, -1), (360, 1), (8192, -1), (65536, -1), (30030, 1)])
+def test_measured_plan_bit_exact(lf, oracle, tmp_path, n, sign):
+    lf.set_wisdom_path(str(tmp_path / "wisdom.jsonl"))
+    try:
+        hm = 3
+        p = lf.Plan(n, sign, hm, flags=lf.MEASURE)
+        rs = p.radices
+        assert int(np.prod(rs)) == n
+        x = oracle.lcg_input(n, seed=n % 97, howmany=hm)
+        assert bits_equal(run_plan(p, x), oracle.ref_transform(sign < 0, rs, x))
+    finally:
+        lf.set_wisdom_path(None)
+
+
+def test_wisdom_round_trip_and_invalid_lines(lf, oracle, tmp_path):
+    path = tmp_path / "wisdom.jsonl"
+    lf.set_wisdom_path(str(path))
+    try:
+        n, hm = 2048, 5
+        p1 = lf.Plan(n, -1, hm, flags=lf.MEASURE)
+        lines = [json.loads(s) for s in path.read_text().splitlines()]
+        assert len(lines) == 1 and lines[0]["rs"] == p1.radices
+        key = lines[0]["key"]
+        # a second plan of the same class reads the cache instead of measuring again
+        p2 = lf.Plan(n, -1, hm, flags=lf.MEASURE)
+        assert p2.radices == p1.radices
+        assert len(path.read_text().splitlines()) == 1
+        # a hand-written line with another valid schedule is obeyed (the last valid line wins)
+        forced = [2] * 11
+        with open(path, "a") as f:
+            f.write(json.dumps({"key": key, "rs": forced, "split": 0, "us": 1.0}) + "\n")
+            # invalid: wrong product, radix 17, split past the end -- all ignored
+            f.write(json.dumps({"key": key, "rs": [4, 4], "split": 0, "us": 0.5}) + "\n")
+            f.write(json.dumps({"key": key, "rs": [17, 2], "split": 0, "us": 0.5}) + "\n")
+            f.write(json.dumps({"key": key, "rs": forced, "split": 11, "us": 0.5}) + "\n")
+        p3 = lf.Plan(n, -1, hm, flags=lf.MEASURE)
+        assert p3.radices == forced
+        x = oracle.lcg_input(n, seed=3, howmany=hm)
+        assert bits_equal(run_plan(p3, x), oracle.ref_transform(True, forced, x))
+    finally:
+        lf.set_wisdom_path(None)
+
+
+def test_measured_strided_plan(lf, oracle, tmp_path):
+    """Transposed layout (istride = howmany, idist = 1) measured on scratch buffers of its own
+    span; the user's arrays are never touched during planning."""
+    lf.set_wisdom_path(str(tmp_path / "wisdom.jsonl"))
+    try:
+        n, hm = 512, 6
+        x = oracle.lcg_input(n, seed=9, howmany=hm)
+        xt = np.ascontiguousarray(x.reshape(hm, n).T).reshape(-1)
+        p = lf.Plan(n, -1, hm, istride=hm, idist=1, ostride=hm, odist=1, flags=lf.MEASURE)
+        xd = torch.from_numpy(xt).to(DEV)
+        y = torch.empty_like(xd)
+        p.call(xd, y)
+        torch.cuda.synchronize()
+        got = y.cpu().numpy().reshape(n, hm).T.reshape(-1)
+        assert bits_equal(got, oracle.ref_transform(True, p.radices, x).reshape(-1))
+    finally:
+        lf.set_wisdom_path(None)
