@@ -42,7 +42,16 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False) -> str:
+def _stale(obj: str, deps: list[str]) -> bool:
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build(verbose: bool = False, force: bool = True) -> str:
+    """Compile every translation unit (in parallel) and link. With force=False, objects newer
+    than their source, the shared headers and this script are reused."""
     sys.path.insert(0, HERE)
     try:
         import codegen  # noqa: WPS433
@@ -50,8 +59,10 @@ def build(verbose: bool = False) -> str:
         sys.path.pop(0)
     codegen.main()
     os.makedirs(OUT, exist_ok=True)
-    objs = []
     cc = hipcc()
+    headers = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    headers += [os.path.join(HERE, "..", "include", "mfft.h"), os.path.abspath(__file__)]
+    jobs, objs = [], []
     for src, device in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(OUT, src.rsplit(".", 1)[0] + ".o")
@@ -59,10 +70,17 @@ def build(verbose: bool = False) -> str:
         cmd = [cc, *flags, "-c", path, "-o", obj, "-I", CSRC]
         if device:
             cmd[1:1] = ["-x", "hip"]
-        if verbose:
-            print(" ".join(cmd))
-        subprocess.run(cmd, check=True)
         objs.append(obj)
+        if force or _stale(obj, [path, *headers]):
+            jobs.append(cmd)
+    procs = []
+    for cmd in jobs:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+    failed = [p.args for p in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
     tmp = LIB + ".tmp"
     cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"]
     subprocess.run(cmd, check=True)
@@ -71,4 +89,4 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, force="--incremental" not in sys.argv))

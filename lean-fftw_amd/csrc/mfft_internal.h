@@ -38,6 +38,10 @@ constexpr int kLdsBytes = (LDS_ELEMS + LDS_ELEMS / 16 + LDS_ELEMS / 2) * 16;
 // the large-tile variants: up to BIG_ELEMS points per workgroup, 16 per thread
 constexpr int BIG_ELEMS = 8192;
 constexpr int kLdsBytesBig = (BIG_ELEMS + BIG_ELEMS / 16 + 1) * 16;
+// dynamic LDS limit of the stage kernels: a tile of up to 8192 points of sub-problems padded
+// one element per 16 plus one each (at most 139,520 bytes), below the 160 KiB of a gfx950 CU
+// with room for the kernels' static LDS
+constexpr int kLdsBytesMax = 144 * 1024;
 
 // One Stockham pass inside a kernel stage. Local index k of butterfly j is j mod nsl (nsl:
 // product of the stage's earlier radices); its twiddle for input r (1 <= r < R) is

@@ -79,7 +79,7 @@ hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const doub
           for (int q = 0; q < 2; ++q)
             for (int b = 0; b < 2; ++b)
               (void)hipFuncSetAttribute(kernel_for(f, st, e, q, b),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytesBig);
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytesMax);
     return true;
   }();
   (void)attr;

@@ -34,6 +34,19 @@ __device__ __forceinline__ double2 twmul(double2 w, double2 x) {
   return make_double2(fma(w.x, x.x, -(w.y * x.y)), fma(w.x, x.y, w.y * x.x));
 }
 
+// Streaming HBM access: transform inputs are read once and outputs written once, so they
+// are loaded and stored non-temporally (the copy rate of MI355X rises from about 5.1 to
+// 6.4 TB/s with this hint, and the end-of-kernel L2 write-back has less to do).
+typedef double d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ld_stream(const double2 *p) {
+  const d2v t = __builtin_nontemporal_load(reinterpret_cast<const d2v *>(p));
+  return make_double2(t.x, t.y);
+}
+__device__ __forceinline__ void st_stream(double2 *p, double2 v) {
+  const d2v t = {v.x, v.y};
+  __builtin_nontemporal_store(t, reinterpret_cast<d2v *>(p));
+}
+
 // q = n / d for n * d < 2^32, with m = ceil(2^32 / d) (m = 0 encodes d = 1)
 __device__ __forceinline__ uint32_t fdiv(uint32_t n, uint32_t m) {
   return m ? __umulhi(n, m) : n;
@@ -92,7 +105,7 @@ __device__ __forceinline__ double pass_run(double2 *buf, const HbmSide &hin, con
         const double2 *s = hin.base + hbm_off(hin, g) + (int64_t)j * hin.stride;
         const int64_t st = (int64_t)LR * hin.stride;
 #pragma unroll
-        for (int r = 0; r < R; ++r) x[u][r] = s[r * st];
+        for (int r = 0; r < R; ++r) x[u][r] = ld_stream(s + r * st);
         if (hin.wt) {
           const double *w = hin.wt + (int64_t)g * hin.wg + (int64_t)j * hin.wstride;
           const int64_t ws = (int64_t)LR * hin.wstride;
@@ -139,7 +152,7 @@ __device__ __forceinline__ double pass_run(double2 *buf, const HbmSide &hin, con
         }
       }
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r * st] = x[u][r];
+      for (int r = 0; r < R; ++r) st_stream(d + r * st, x[u][r]);
       if constexpr (SQ) {
 #pragma unroll
         for (int r = 0; r < R; ++r) sumsq = sumsq + (x[u][r].x * x[u][r].x + x[u][r].y * x[u][r].y);
@@ -165,7 +178,7 @@ __device__ __forceinline__ void tile_to_lds(double2 *buf, const HbmSide &h, int 
   for (int u = 0; u < EPT; ++u) {
     const int f = tid + u * nthreads;
     if (f < total) {
-      v[u] = h.base[f];
+      v[u] = ld_stream(h.base + f);
       if (h.wt) {
         const int g = (int)fdiv((uint32_t)f, m_L);
         const double w = h.wt[(int64_t)g * h.wg + (int64_t)(f - g * L) * h.wstride];
@@ -201,7 +214,7 @@ __device__ __forceinline__ double tile_from_lds(const double2 *buf, const HbmSid
         v.x = v.x * w;
         v.y = v.y * w;
       }
-      h.base[f] = v;
+      st_stream(h.base + f, v);
       if constexpr (SQ) sumsq = sumsq + (v.x * v.x + v.y * v.y);
     }
   }
