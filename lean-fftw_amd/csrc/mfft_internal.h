@@ -68,6 +68,8 @@ struct StageArgs {
                 // G*L <= BIG_ELEMS, up to 512 threads)
   int gfast_in, gfast_out;  // 1: consecutive threads walk g (g adjacent in memory)
   int stage_io;             // 1: coalesced tile copies HBM -> LDS and back (contiguous small L)
+  int reg;                  // 1: register-resident kernel (mfft_reg.hip), one transform of
+                            // L = 2048..16384 points per workgroup of L / 16 threads
   uint32_t m_L;             // ceil(2^32 / L), for the tile copies
   PassArgs pass[MAX_PASSES];
   int64_t nsub;
@@ -92,6 +94,12 @@ struct RClass {
 // Describe a single-transform plan as a ragged size class, appending its twiddle table
 // to `tw` (offsets in the class are relative to the start of `tw`). Returns 0 on success.
 int export_class(const struct ::mfft_plan *p, RClass *c, std::vector<double2> *tw);
+
+// The register-resident kernel: power-of-two n in [2048, 8192] with radices in {2, 4, 8}, or
+// n = 16384 as radix 4 only.
+bool reg_supported(int64_t n, const int *radices, int nradix);
+hipError_t launch_reg(bool fwd, const StageArgs &a, const double2 *in, double2 *out,
+                      const double2 *tw, hipStream_t stream);
 
 // Launch one stage over `ntrans` transforms; returns a hipError_t.
 hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,

@@ -152,6 +152,15 @@ std::vector<int> choose_radices(int64_t L) {
   return all;
 }
 
+// Radix sequence of a register-resident size: radices up to 8 with 8 elements per thread
+// (N <= 8192), radix 4 only with 16 (N = 16384).
+std::vector<int> reg_radices(int64_t n) {
+  std::vector<int> rs;
+  double c;
+  best_radices(n, any_radix, rs, c, n > 8192 ? 4 : 8);
+  return rs;
+}
+
 // Split N into P * Q for a two-stage plan: both within the LDS, as balanced as possible.
 bool choose_split(int64_t N, int64_t &P) {
   P = 0;
@@ -167,6 +176,27 @@ bool choose_split(int64_t N, int64_t &P) {
     }
   }
   return P != 0;
+}
+
+// Power-of-two sizes that run as one register-resident stage (mfft_reg.hip): from
+// MFFT_REG_MIN (default 2048) to 16384 points; MFFT_REG=0 turns the kernel off.
+int64_t reg_min() {
+  static const int64_t v = [] {
+    const char *e = std::getenv("MFFT_REG");
+    if (e && std::atoi(e) == 0) return (int64_t)1 << 40;
+    const char *m = std::getenv("MFFT_REG_MIN");
+    return m ? (int64_t)std::atoll(m) : (int64_t)2048;
+  }();
+  return v;
+}
+
+bool reg_size(int64_t n) { return (n & (n - 1)) == 0 && n >= reg_min() && n <= 16384; }
+
+// Whether a size runs as a single kernel stage (else it is split into two).
+bool single_stage(int64_t n, bool allow_big) {
+  const bool pow2 = (n & (n - 1)) == 0;
+  return n <= mfft::LDS_ELEMS ||
+         (allow_big && (n <= (pow2 ? mfft::BIG_ELEMS : mfft::BIG_ELEMS / 2) || reg_size(n)));
 }
 
 hipError_t build_twiddles(mfft_plan *p) {
@@ -271,9 +301,10 @@ bool build_stages(mfft_plan *p, int split) {
   std::vector<int64_t> ns(s + 1, 1);
   for (int t = 0; t < s; ++t) ns[t + 1] = ns[t] * rs[t];
   if (split <= 0 || split >= s) {
-    const bool big = N > mfft::LDS_ELEMS && p->allow_big &&
+    const bool reg = p->allow_big && reg_size(N) && mfft::reg_supported(N, rs.data(), s);
+    const bool big = !reg && N > mfft::LDS_ELEMS && p->allow_big &&
                      N <= ((N & (N - 1)) == 0 ? mfft::BIG_ELEMS : mfft::BIG_ELEMS / 2);
-    if ((N > mfft::LDS_ELEMS && !big) || s > mfft::MAX_PASSES) return false;
+    if ((N > mfft::LDS_ELEMS && !big && !reg) || s > mfft::MAX_PASSES) return false;
     StageArgs a{};
     a.L = (int)N;
     a.npass = s;
@@ -300,6 +331,13 @@ bool build_stages(mfft_plan *p, int split) {
                   p->idist == N && p->odist == N)
                      ? 1
                      : 0;
+    if (reg) {  // one transform per workgroup, 16 elements per thread
+      a.reg = 1;
+      a.G = 1;
+      a.ept = 16;
+      a.threads = (int)(N / 16);
+      a.stage_io = 0;
+    }
     p->stages.push_back(a);
     p->stage_passes.push_back(s);
     p->chunk_h = p->howmany;
@@ -447,7 +485,7 @@ std::vector<std::vector<int>> seq_candidates(int64_t L) {
 std::vector<Cand> schedule_candidates(int64_t n, bool allow_big) {
   std::vector<Cand> out;
   const bool pow2 = (n & (n - 1)) == 0;
-  if (n <= mfft::LDS_ELEMS || (allow_big && n <= (pow2 ? mfft::BIG_ELEMS : mfft::BIG_ELEMS / 2)))
+  if (single_stage(n, allow_big))
     for (auto &v : seq_candidates(n)) out.push_back({v, 0});
   if (n > mfft::LDS_ELEMS) {
     std::vector<std::pair<double, int64_t>> splits;
@@ -644,9 +682,8 @@ mfft_plan *plan_many_internal(int64_t n, int sign, int64_t howmany, int64_t istr
       }
     }
   }
-  if (n <= mfft::LDS_ELEMS ||
-      (allow_big && n <= (pow2 ? mfft::BIG_ELEMS : mfft::BIG_ELEMS / 2))) {
-    p->radices = choose_radices(n);
+  if (single_stage(n, allow_big)) {
+    p->radices = allow_big && reg_size(n) ? reg_radices(n) : choose_radices(n);
   } else {
     int64_t P;
     if (!choose_split(n, P)) {
@@ -720,8 +757,8 @@ int mfft_plan_schedule(int64_t n, int *radices, int max, int *split) {
   std::vector<int> rs;
   int sp = 0;
   const bool pow2 = (n & (n - 1)) == 0;
-  if (n <= mfft::LDS_ELEMS || n <= (pow2 ? mfft::BIG_ELEMS : mfft::BIG_ELEMS / 2)) {
-    rs = choose_radices(n);
+  if (single_stage(n, true)) {
+    rs = reg_size(n) ? reg_radices(n) : choose_radices(n);
   } else {
     int64_t P;
     if (!choose_split(n, P)) return -1;

@@ -1,0 +1,211 @@
+// Register-resident Stockham kernel for gfx950: one power-of-two transform of N points per
+// workgroup, T = N / E threads, E elements per thread (8, or 16 at N = 16384) held in
+// registers for the whole transform. Used for N = 2048 ... 16384, where a transform is too
+// large for several LDS-resident tiles per CU (16384 points are 256 KiB, more than the
+// 160 KiB of LDS) but small enough for the registers of one CU.
+//
+// Every pass is the reference's passWith (LeanFFTW/Gen/Stockham.lean:88-98) with the
+// butterflies of csrc/bfly_gen.h and the plan's pass tables, so the result is bit-identical
+// to refTransform for the same radix sequence. Thread t always holds the elements
+// t + T m (m < E): for radix R its butterflies are j = t + T u (u < E / R), whose inputs
+// x[j + r N/R] are exactly those elements (m = u + r E/R). So the first pass reads HBM and
+// the last pass writes HBM with consecutive lanes on consecutive addresses, and between
+// passes the outputs go to LDS at their Stockham positions and come back as the next
+// pass's columns. The exchange moves the real parts, then the imaginary parts, through one
+// buffer of N doubles (128 KiB at N = 16384), which halves the LDS a transform needs.
+#include <mutex>
+#include <set>
+
+#include "mfft_pass.h"
+
+namespace mfft {
+
+// LDS index of double e: one padding word after every 32, so that outputs written at a
+// stride of R <= 16 doubles by consecutive lanes spread over the banks.
+__device__ __forceinline__ int rpad(int e) { return e + (e >> 5); }
+__host__ __device__ constexpr int reg_lds_doubles(int n) { return n + (n >> 5) + 1; }
+
+// One pass of radix R on the E registers v, in place: butterfly u (u < E / R) takes its
+// inputs from registers u + r E/R (columns t + T (u + r E/R)) and leaves output r there.
+template <bool FWD, int R, int T, int E>
+__device__ __forceinline__ void reg_bfly(double2 (&v)[E], int t, int ns,
+                                         const double2 *__restrict__ twp, bool twd) {
+  constexpr int NB = E / R;
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    double2 x[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = v[u + r * NB];
+    if (twd) {
+      const int k = (t + u * T) & (ns - 1);
+#pragma unroll
+      for (int r = 1; r < R; ++r) x[r] = twmul(twp[(r - 1) * ns + k], x[r]);
+    }
+    Bfly<FWD, R, false>::run(x);
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[u + r * NB] = x[r];
+    // one butterfly's twiddles in flight at a time: hoisting the next butterfly's loads
+    // above this one costs registers that 16 waves per CU do not have
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Exchange through LDS after a pass of radix R: output r of butterfly u (register
+// u + r E/R) goes to its Stockham position (j / ns) ns R + j mod ns + r ns, j = t + T u;
+// then every thread reads its columns t + T m.
+template <int R, int T, int E>
+__device__ __forceinline__ void reg_exchange(double2 (&v)[E], double *lds, int t, int ns) {
+  constexpr int NB = E / R;
+  int pos[E];
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    const int j = t + u * T;
+    const int k = j & (ns - 1);
+    const int base = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) pos[u + r * NB] = rpad(base + r * ns);
+  }
+#pragma unroll
+  for (int i = 0; i < E; ++i) lds[pos[i]] = v[i].x;
+  __syncthreads();
+  // T is a multiple of 32, so rpad(t + T m) = rpad(t) + m (T + T / 32): fixed offsets
+  const double *col = lds + rpad(t);
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m].x = col[m * (T + T / 32)];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < E; ++i) lds[pos[i]] = v[i].y;
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m].y = col[m * (T + T / 32)];
+  __syncthreads();
+}
+
+// T = N / E threads; registers held to 128 per thread so that 16 waves fit on a CU.
+template <bool FWD, int T, int E, bool SQ>
+__global__ __launch_bounds__(T, 4) void reg_kernel(const double2 *in, double2 *out,
+                                                   const double2 *__restrict__ tw,
+                                                   const StageArgs a) {
+  extern __shared__ double lds_d[];
+  const int t = threadIdx.x;
+  const int64_t h = blockIdx.x;
+  const double2 *src = in + h * a.in_g;
+  double2 *dst = out + h * a.out_g;
+  double2 v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = ld_stream(src + (int64_t)(t + T * m) * a.in_e);
+  if (a.pre) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const double w = a.pre[(int64_t)(t + T * m) * a.pre_e];
+      v[m].x = v[m].x * w;
+      v[m].y = v[m].y * w;
+    }
+  }
+  for (int p = 0; p < a.npass; ++p) {
+    const PassArgs pa = a.pass[p];
+    const bool twd = pa.twoff >= 0;
+    const double2 *twp = tw + (twd ? pa.twoff : 0);
+    const bool last = p == a.npass - 1;
+#define MFFT_REG(RR)                                              \
+  case RR:                                                        \
+    reg_bfly<FWD, RR, T, E>(v, t, pa.nsl, twp, twd);              \
+    if (!last) reg_exchange<RR, T, E>(v, lds_d, t, pa.nsl);       \
+    break;
+    // 16 elements per thread (N = 16384) run radix 4 only: radix 2's eight butterflies per
+    // thread, or radix 8's larger ones, exceed the 128 registers of 16 waves per CU
+    switch (pa.R) {
+      case 2:
+        if constexpr (E == 8) {
+          reg_bfly<FWD, 2, T, E>(v, t, pa.nsl, twp, twd);
+          if (!last) reg_exchange<2, T, E>(v, lds_d, t, pa.nsl);
+        }
+        break;
+      MFFT_REG(4)
+      case 8:
+        if constexpr (E == 8) {
+          reg_bfly<FWD, 8, T, E>(v, t, pa.nsl, twp, twd);
+          if (!last) reg_exchange<8, T, E>(v, lds_d, t, pa.nsl);
+        }
+        break;
+      default:
+        break;
+    }
+#undef MFFT_REG
+  }
+  // the last pass (ns = N / R) writes output r of butterfly j = t + T u to j + r N/R, the
+  // column of the register that holds it: register m goes to element t + T m. (The opaque
+  // copy of t keeps the compiler from computing these addresses before the pass loop and
+  // holding them in registers through it.)
+  int ts = t;
+  asm volatile("" : "+v"(ts));
+  double sumsq = 0.0;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int e = ts + T * m;
+    double2 y = v[m];
+    if (a.post) {
+      const double w = a.post[(int64_t)e * a.post_e];
+      y.x = y.x * w;
+      y.y = y.y * w;
+    }
+    st_stream(dst + (int64_t)e * a.out_e, y);
+    if constexpr (SQ) sumsq = sumsq + (y.x * y.x + y.y * y.y);
+  }
+  if constexpr (SQ) {
+    __shared__ double wsum[T / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sumsq = sumsq + __shfl_xor(sumsq, o, 64);
+    if ((t & 63) == 0) wsum[t >> 6] = sumsq;
+    __syncthreads();
+    if (t == 0) {
+      double s = 0.0;
+      for (int w = 0; w < T / 64; ++w) s = s + wsum[w];
+      a.sumsq[h] = s;
+    }
+  }
+}
+
+// Elements per thread: 8 (T = N / 8 <= 1024 threads) up to 8192 points, 16 at 16384.
+constexpr int reg_elems(int64_t n) { return n <= 8192 ? 8 : 16; }
+
+bool reg_supported(int64_t n, const int *radices, int nradix) {
+  if (n < 2048 || n > 16384 || (n & (n - 1)) || nradix < 1 || nradix > MAX_PASSES) return false;
+  for (int i = 0; i < nradix; ++i) {
+    const int r = radices[i];
+    if (reg_elems(n) == 16 ? r != 4 : (r != 2 && r != 4 && r != 8)) return false;
+  }
+  return true;
+}
+
+hipError_t launch_reg(bool fwd, const StageArgs &a, const double2 *in, double2 *out,
+                      const double2 *tw, hipStream_t stream) {
+  if (a.nsub <= 0) return hipSuccess;
+  const int E = reg_elems(a.L), T = a.L / E;
+  const size_t lds = (size_t)reg_lds_doubles(a.L) * sizeof(double);
+  const bool sq = a.sumsq != nullptr;
+  const void *k = nullptr;
+#define MFFT_RK(TT, EE)                                                        \
+  (fwd ? (sq ? (const void *)reg_kernel<true, TT, EE, true>                    \
+             : (const void *)reg_kernel<true, TT, EE, false>)                  \
+       : (sq ? (const void *)reg_kernel<false, TT, EE, true>                   \
+             : (const void *)reg_kernel<false, TT, EE, false>))
+  if (E == 8 && T == 256) k = MFFT_RK(256, 8);
+  else if (E == 8 && T == 512) k = MFFT_RK(512, 8);
+  else if (E == 8 && T == 1024) k = MFFT_RK(1024, 8);
+  else if (E == 16 && T == 1024) k = MFFT_RK(1024, 16);
+  else return hipErrorInvalidValue;
+#undef MFFT_RK
+  {
+    static std::mutex mu;
+    static std::set<const void *> done;
+    std::lock_guard<std::mutex> lk(mu);
+    if (done.insert(k).second)
+      (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytesMax);
+  }
+  void *args[] = {(void *)&in, (void *)&out, (void *)&tw, (void *)&a};
+  (void)hipLaunchKernel(k, dim3((unsigned)a.nsub), dim3(T), args, lds, stream);
+  return hipGetLastError();
+}
+
+}  // namespace mfft
