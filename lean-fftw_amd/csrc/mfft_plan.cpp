@@ -157,7 +157,7 @@ std::vector<int> choose_radices(int64_t L) {
 std::vector<int> reg_radices(int64_t n) {
   std::vector<int> rs;
   double c;
-  best_radices(n, any_radix, rs, c, n > 8192 ? 4 : 8);
+  best_radices(n, any_radix, rs, c, mfft::reg_elems(n) == 16 ? 4 : 8);
   return rs;
 }
 
@@ -179,13 +179,13 @@ bool choose_split(int64_t N, int64_t &P) {
 }
 
 // Power-of-two sizes that run as one register-resident stage (mfft_reg.hip): from
-// MFFT_REG_MIN (default 2048) to 16384 points; MFFT_REG=0 turns the kernel off.
+// MFFT_REG_MIN (default 512) to 16384 points; MFFT_REG=0 turns the kernel off.
 int64_t reg_min() {
   static const int64_t v = [] {
     const char *e = std::getenv("MFFT_REG");
     if (e && std::atoi(e) == 0) return (int64_t)1 << 40;
     const char *m = std::getenv("MFFT_REG_MIN");
-    return m ? (int64_t)std::atoll(m) : (int64_t)2048;
+    return m ? (int64_t)std::atoll(m) : (int64_t)512;
   }();
   return v;
 }

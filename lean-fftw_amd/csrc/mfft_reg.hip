@@ -1,8 +1,8 @@
 // Register-resident Stockham kernel for gfx950: one power-of-two transform of N points per
 // workgroup, T = N / E threads, E elements per thread (8, or 16 at N = 16384) held in
-// registers for the whole transform. Used for N = 2048 ... 16384, where a transform is too
-// large for several LDS-resident tiles per CU (16384 points are 256 KiB, more than the
-// 160 KiB of LDS) but small enough for the registers of one CU.
+// registers for the whole transform. Used for N = 512 ... 16384: up to 8192 points it beats
+// the LDS-tile kernel, and at 16384 a transform (256 KiB, more than the 160 KiB of LDS)
+// still fits the registers of one CU, so it runs in one HBM round trip.
 //
 // Every pass is the reference's passWith (LeanFFTW/Gen/Stockham.lean:88-98) with the
 // butterflies of csrc/bfly_gen.h and the plan's pass tables, so the result is bit-identical
@@ -11,8 +11,10 @@
 // x[j + r N/R] are exactly those elements (m = u + r E/R). So the first pass reads HBM and
 // the last pass writes HBM with consecutive lanes on consecutive addresses, and between
 // passes the outputs go to LDS at their Stockham positions and come back as the next
-// pass's columns. The exchange moves the real parts, then the imaginary parts, through one
-// buffer of N doubles (128 KiB at N = 16384), which halves the LDS a transform needs.
+// pass's columns. Up to 8192 points the exchange moves whole complex values (N padded
+// double2, 136 KiB at 8192); at 16384 it moves the real parts, then the imaginary parts,
+// through one buffer of N doubles (132 KiB), half of what the transform occupies.
+#include <cstdlib>
 #include <mutex>
 #include <set>
 
@@ -82,11 +84,43 @@ __device__ __forceinline__ void reg_exchange(double2 (&v)[E], double *lds, int t
 }
 
 // T = N / E threads; registers held to 128 per thread so that 16 waves fit on a CU.
+// The same exchange with whole complex values through N padded double2 (two barriers).
+template <int R, int T, int E>
+__device__ __forceinline__ void reg_exchange_full(double2 (&v)[E], double2 *lds, int t, int ns) {
+  constexpr int NB = E / R;
+  int pos[E];
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    const int j = t + u * T;
+    const int k = j & (ns - 1);
+    const int base = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) pos[u + r * NB] = lpad(base + r * ns);
+  }
+#pragma unroll
+  for (int i = 0; i < E; ++i) lds[pos[i]] = v[i];
+  __syncthreads();
+  // T is a multiple of 16, so lpad(t + T m) = lpad(t) + m (T + T / 16)
+  const double2 *col = lds + lpad(t);
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = col[m * (T + T / 16)];
+  __syncthreads();
+}
+
+template <int R, int T, int E>
+__device__ __forceinline__ void reg_exchange_any(double2 (&v)[E], void *lds, int t, int ns) {
+  if constexpr (E == 8)
+    reg_exchange_full<R, T, E>(v, static_cast<double2 *>(lds), t, ns);
+  else
+    reg_exchange<R, T, E>(v, static_cast<double *>(lds), t, ns);
+}
+
 template <bool FWD, int T, int E, bool SQ>
 __global__ __launch_bounds__(T, 4) void reg_kernel(const double2 *in, double2 *out,
                                                    const double2 *__restrict__ tw,
                                                    const StageArgs a) {
-  extern __shared__ double lds_d[];
+  extern __shared__ double2 lds_c[];
+  void *lds_d = lds_c;
   const int t = threadIdx.x;
   const int64_t h = blockIdx.x;
   const double2 *src = in + h * a.in_g;
@@ -110,7 +144,7 @@ __global__ __launch_bounds__(T, 4) void reg_kernel(const double2 *in, double2 *o
 #define MFFT_REG(RR)                                              \
   case RR:                                                        \
     reg_bfly<FWD, RR, T, E>(v, t, pa.nsl, twp, twd);              \
-    if (!last) reg_exchange<RR, T, E>(v, lds_d, t, pa.nsl);       \
+    if (!last) reg_exchange_any<RR, T, E>(v, lds_d, t, pa.nsl);   \
     break;
     // 16 elements per thread (N = 16384) run radix 4 only: radix 2's eight butterflies per
     // thread, or radix 8's larger ones, exceed the 128 registers of 16 waves per CU
@@ -118,14 +152,14 @@ __global__ __launch_bounds__(T, 4) void reg_kernel(const double2 *in, double2 *o
       case 2:
         if constexpr (E == 8) {
           reg_bfly<FWD, 2, T, E>(v, t, pa.nsl, twp, twd);
-          if (!last) reg_exchange<2, T, E>(v, lds_d, t, pa.nsl);
+          if (!last) reg_exchange_any<2, T, E>(v, lds_d, t, pa.nsl);
         }
         break;
       MFFT_REG(4)
       case 8:
         if constexpr (E == 8) {
           reg_bfly<FWD, 8, T, E>(v, t, pa.nsl, twp, twd);
-          if (!last) reg_exchange<8, T, E>(v, lds_d, t, pa.nsl);
+          if (!last) reg_exchange_any<8, T, E>(v, lds_d, t, pa.nsl);
         }
         break;
       default:
@@ -166,11 +200,19 @@ __global__ __launch_bounds__(T, 4) void reg_kernel(const double2 *in, double2 *o
   }
 }
 
-// Elements per thread: 8 (T = N / 8 <= 1024 threads) up to 8192 points, 16 at 16384.
-constexpr int reg_elems(int64_t n) { return n <= 8192 ? 8 : 16; }
+// Elements per thread: 16 (radix 4 only) for powers of 4 from MFFT_REG_E16 (default 16384)
+// on, else 8 (T = N / 8 <= 1024 threads, up to 8192 points).
+int reg_elems(int64_t n) {
+  static const int64_t e16 = [] {
+    const char *e = std::getenv("MFFT_REG_E16");
+    return e ? (int64_t)std::atoll(e) : (int64_t)16384;
+  }();
+  const bool pow4 = (__builtin_ctzll((uint64_t)n) & 1) == 0;
+  return (n > 8192 || (n >= e16 && pow4)) ? 16 : 8;
+}
 
 bool reg_supported(int64_t n, const int *radices, int nradix) {
-  if (n < 2048 || n > 16384 || (n & (n - 1)) || nradix < 1 || nradix > MAX_PASSES) return false;
+  if (n < 512 || n > 16384 || (n & (n - 1)) || nradix < 1 || nradix > MAX_PASSES) return false;
   for (int i = 0; i < nradix; ++i) {
     const int r = radices[i];
     if (reg_elems(n) == 16 ? r != 4 : (r != 2 && r != 4 && r != 8)) return false;
@@ -182,7 +224,8 @@ hipError_t launch_reg(bool fwd, const StageArgs &a, const double2 *in, double2 *
                       const double2 *tw, hipStream_t stream) {
   if (a.nsub <= 0) return hipSuccess;
   const int E = reg_elems(a.L), T = a.L / E;
-  const size_t lds = (size_t)reg_lds_doubles(a.L) * sizeof(double);
+  const size_t lds = E == 8 ? (size_t)lds_len(a.L) * sizeof(double2)
+                            : (size_t)reg_lds_doubles(a.L) * sizeof(double);
   const bool sq = a.sumsq != nullptr;
   const void *k = nullptr;
 #define MFFT_RK(TT, EE)                                                        \
@@ -190,9 +233,13 @@ hipError_t launch_reg(bool fwd, const StageArgs &a, const double2 *in, double2 *
              : (const void *)reg_kernel<true, TT, EE, false>)                  \
        : (sq ? (const void *)reg_kernel<false, TT, EE, true>                   \
              : (const void *)reg_kernel<false, TT, EE, false>))
-  if (E == 8 && T == 256) k = MFFT_RK(256, 8);
+  if (E == 8 && T == 64) k = MFFT_RK(64, 8);
+  else if (E == 8 && T == 128) k = MFFT_RK(128, 8);
+  else if (E == 8 && T == 256) k = MFFT_RK(256, 8);
   else if (E == 8 && T == 512) k = MFFT_RK(512, 8);
   else if (E == 8 && T == 1024) k = MFFT_RK(1024, 8);
+  else if (E == 16 && T == 64) k = MFFT_RK(64, 16);
+  else if (E == 16 && T == 256) k = MFFT_RK(256, 16);
   else if (E == 16 && T == 1024) k = MFFT_RK(1024, 16);
   else return hipErrorInvalidValue;
 #undef MFFT_RK
