@@ -214,7 +214,7 @@ def bench_mixed(args, torch, lf):
         hm = -(-(1 << 24) // N)
         x = uniform_complex(torch, (hm, N), N)
         y = torch.empty_like(x)
-        p = lf.Plan(N, lf.FORWARD, hm)
+        p = lf.Plan(N, lf.FORWARD, hm, flags=lf.MEASURE if args.measure else lf.ESTIMATE)
         t = time_steps(torch, lambda: p.call(x, y), args.steps, args.warmup)
         b = 32.0 * N * hm
         per.append({"N": N, "batch": hm, "radices": p.radices, "stages": p.stages,
