@@ -69,7 +69,7 @@ struct StageArgs {
   int gfast_in, gfast_out;  // 1: consecutive threads walk g (g adjacent in memory)
   int stage_io;             // 1: coalesced tile copies HBM -> LDS and back (contiguous small L)
   int reg;                  // 1: register-resident kernel (mfft_reg.hip), one transform of
-                            // L = 512..16384 points per workgroup of L / 8 or L / 16 threads
+                            // L = 32..16384 points, L / 8 or L / 16 threads per transform
   uint32_t m_L;             // ceil(2^32 / L), for the tile copies
   PassArgs pass[MAX_PASSES];
   int64_t nsub;
@@ -95,7 +95,7 @@ struct RClass {
 // to `tw` (offsets in the class are relative to the start of `tw`). Returns 0 on success.
 int export_class(const struct ::mfft_plan *p, RClass *c, std::vector<double2> *tw);
 
-// The register-resident kernel: power-of-two n in [512, 8192] with radices in {2, 4, 8}, or
+// The register-resident kernel: power-of-two n in [32, 8192] with radices in {2, 4, 8}, or
 // n = 16384 as radix 4 only.
 bool reg_supported(int64_t n, const int *radices, int nradix);
 int reg_elems(int64_t n);  // elements per thread of the register-resident kernel at size n

@@ -179,13 +179,13 @@ bool choose_split(int64_t N, int64_t &P) {
 }
 
 // Power-of-two sizes that run as one register-resident stage (mfft_reg.hip): from
-// MFFT_REG_MIN (default 512) to 16384 points; MFFT_REG=0 turns the kernel off.
+// MFFT_REG_MIN (default 64) to 16384 points; MFFT_REG=0 turns the kernel off.
 int64_t reg_min() {
   static const int64_t v = [] {
     const char *e = std::getenv("MFFT_REG");
     if (e && std::atoi(e) == 0) return (int64_t)1 << 40;
     const char *m = std::getenv("MFFT_REG_MIN");
-    return m ? (int64_t)std::atoll(m) : (int64_t)512;
+    return m ? (int64_t)std::atoll(m) : (int64_t)64;
   }();
   return v;
 }
@@ -332,11 +332,12 @@ bool build_stages(mfft_plan *p, int split) {
                   p->idist == N && p->odist == N)
                      ? 1
                      : 0;
-    if (reg) {  // one transform per workgroup, 16 elements per thread
+    if (reg) {  // T = N / E threads per transform, 256 / T transforms per workgroup if T < 256
+      const int E = mfft::reg_elems(N), T = (int)(N / E);
       a.reg = 1;
-      a.G = 1;
-      a.ept = 16;
-      a.threads = (int)(N / 16);
+      a.ept = E;
+      a.threads = T < 256 ? 256 : T;
+      a.G = a.threads / T;
       a.stage_io = 0;
     }
     p->stages.push_back(a);

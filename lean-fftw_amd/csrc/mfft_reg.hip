@@ -1,6 +1,6 @@
-// Register-resident Stockham kernel for gfx950: one power-of-two transform of N points per
-// workgroup, T = N / E threads, E elements per thread (8, or 16 at N = 16384) held in
-// registers for the whole transform. Used for N = 512 ... 16384: up to 8192 points it beats
+// Register-resident Stockham kernel for gfx950: power-of-two transforms of N points, one per
+// transform, T = N / E threads, E elements per thread (8, or 16 at N = 16384) held in
+// registers for the whole transform. Used for N = 64 ... 16384: up to 8192 points it beats
 // the LDS-tile kernel, and at 16384 a transform (256 KiB, more than the 160 KiB of LDS)
 // still fits the registers of one CU, so it runs in one HBM round trip.
 //
@@ -14,7 +14,6 @@
 // pass's columns. Up to 8192 points the exchange moves whole complex values (N padded
 // double2, 136 KiB at 8192); at 16384 it moves the real parts, then the imaginary parts,
 // through one buffer of N doubles (132 KiB), half of what the transform occupies.
-#include <cstdlib>
 #include <mutex>
 #include <set>
 
@@ -100,10 +99,14 @@ __device__ __forceinline__ void reg_exchange_full(double2 (&v)[E], double2 *lds,
 #pragma unroll
   for (int i = 0; i < E; ++i) lds[pos[i]] = v[i];
   __syncthreads();
-  // T is a multiple of 16, so lpad(t + T m) = lpad(t) + m (T + T / 16)
-  const double2 *col = lds + lpad(t);
+  if constexpr (T % 16 == 0) {  // lpad(t + T m) = lpad(t) + m (T + T / 16)
+    const double2 *col = lds + lpad(t);
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = col[m * (T + T / 16)];
+    for (int m = 0; m < E; ++m) v[m] = col[m * (T + T / 16)];
+  } else {
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = lds[lpad(t + T * m)];
+  }
   __syncthreads();
 }
 
@@ -115,19 +118,33 @@ __device__ __forceinline__ void reg_exchange_any(double2 (&v)[E], void *lds, int
     reg_exchange<R, T, E>(v, static_cast<double *>(lds), t, ns);
 }
 
+// Workgroup size: T threads per transform, and G = 256 / T transforms per workgroup when a
+// transform has fewer than 256 threads (N < 2048).
+template <int T>
+constexpr int reg_block() {
+  return T < 256 ? 256 : T;
+}
+
 template <bool FWD, int T, int E, bool SQ>
-__global__ __launch_bounds__(T, 4) void reg_kernel(const double2 *in, double2 *out,
-                                                   const double2 *__restrict__ tw,
-                                                   const StageArgs a) {
+__global__ __launch_bounds__(reg_block<T>(), 4) void reg_kernel(const double2 *in, double2 *out,
+                                                               const double2 *__restrict__ tw,
+                                                               const StageArgs a) {
+  constexpr int G = reg_block<T>() / T;
+  static_assert(G == 1 || E == 8, "several transforms per workgroup use the whole-complex exchange");
   extern __shared__ double2 lds_c[];
-  void *lds_d = lds_c;
-  const int t = threadIdx.x;
-  const int64_t h = blockIdx.x;
-  const double2 *src = in + h * a.in_g;
-  double2 *dst = out + h * a.out_g;
+  const int t = G == 1 ? (int)threadIdx.x : (int)threadIdx.x % T;
+  const int g = G == 1 ? 0 : (int)threadIdx.x / T;
+  // transform g of the workgroup: its own LDS region; a transform past the batch runs the
+  // passes on zeros and stores nothing
+  void *lds_d = lds_c + g * lds_len(T * E);
+  const int64_t h = (int64_t)blockIdx.x * G + g;
+  const bool valid = h < a.nsub;
+  const double2 *src = in + (valid ? h : 0) * a.in_g;
+  double2 *dst = out + (valid ? h : 0) * a.out_g;
   double2 v[E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = ld_stream(src + (int64_t)(t + T * m) * a.in_e);
+  for (int m = 0; m < E; ++m)
+    v[m] = valid ? ld_stream(src + (int64_t)(t + T * m) * a.in_e) : make_double2(0.0, 0.0);
   if (a.pre) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -183,36 +200,29 @@ __global__ __launch_bounds__(T, 4) void reg_kernel(const double2 *in, double2 *o
       y.x = y.x * w;
       y.y = y.y * w;
     }
-    st_stream(dst + (int64_t)e * a.out_e, y);
+    if (valid) st_stream(dst + (int64_t)e * a.out_e, y);
     if constexpr (SQ) sumsq = sumsq + (y.x * y.x + y.y * y.y);
   }
-  if constexpr (SQ) {
-    __shared__ double wsum[T / 64];
+  if constexpr (SQ) {  // one partial per workgroup, a fixed reduction tree
+    constexpr int BT = reg_block<T>();
+    __shared__ double wsum[BT / 64];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) sumsq = sumsq + __shfl_xor(sumsq, o, 64);
-    if ((t & 63) == 0) wsum[t >> 6] = sumsq;
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = sumsq;
     __syncthreads();
-    if (t == 0) {
+    if (threadIdx.x == 0) {
       double s = 0.0;
-      for (int w = 0; w < T / 64; ++w) s = s + wsum[w];
-      a.sumsq[h] = s;
+      for (int w = 0; w < BT / 64; ++w) s = s + wsum[w];
+      a.sumsq[blockIdx.x] = s;
     }
   }
 }
 
-// Elements per thread: 16 (radix 4 only) for powers of 4 from MFFT_REG_E16 (default 16384)
-// on, else 8 (T = N / 8 <= 1024 threads, up to 8192 points).
-int reg_elems(int64_t n) {
-  static const int64_t e16 = [] {
-    const char *e = std::getenv("MFFT_REG_E16");
-    return e ? (int64_t)std::atoll(e) : (int64_t)16384;
-  }();
-  const bool pow4 = (__builtin_ctzll((uint64_t)n) & 1) == 0;
-  return (n > 8192 || (n >= e16 && pow4)) ? 16 : 8;
-}
+// Elements per thread: 8 (T = N / 8 <= 1024 threads) up to 8192 points, 16 at 16384.
+int reg_elems(int64_t n) { return n > 8192 ? 16 : 8; }
 
 bool reg_supported(int64_t n, const int *radices, int nradix) {
-  if (n < 512 || n > 16384 || (n & (n - 1)) || nradix < 1 || nradix > MAX_PASSES) return false;
+  if (n < 32 || n > 16384 || (n & (n - 1)) || nradix < 1 || nradix > MAX_PASSES) return false;
   for (int i = 0; i < nradix; ++i) {
     const int r = radices[i];
     if (reg_elems(n) == 16 ? r != 4 : (r != 2 && r != 4 && r != 8)) return false;
@@ -224,7 +234,8 @@ hipError_t launch_reg(bool fwd, const StageArgs &a, const double2 *in, double2 *
                       const double2 *tw, hipStream_t stream) {
   if (a.nsub <= 0) return hipSuccess;
   const int E = reg_elems(a.L), T = a.L / E;
-  const size_t lds = E == 8 ? (size_t)lds_len(a.L) * sizeof(double2)
+  const int BT = T < 256 ? 256 : T, G = BT / T;  // as reg_block<T>()
+  const size_t lds = E == 8 ? (size_t)G * lds_len(a.L) * sizeof(double2)
                             : (size_t)reg_lds_doubles(a.L) * sizeof(double);
   const bool sq = a.sumsq != nullptr;
   const void *k = nullptr;
@@ -233,13 +244,15 @@ hipError_t launch_reg(bool fwd, const StageArgs &a, const double2 *in, double2 *
              : (const void *)reg_kernel<true, TT, EE, false>)                  \
        : (sq ? (const void *)reg_kernel<false, TT, EE, true>                   \
              : (const void *)reg_kernel<false, TT, EE, false>))
-  if (E == 8 && T == 64) k = MFFT_RK(64, 8);
+  if (E == 8 && T == 4) k = MFFT_RK(4, 8);
+  else if (E == 8 && T == 8) k = MFFT_RK(8, 8);
+  else if (E == 8 && T == 16) k = MFFT_RK(16, 8);
+  else if (E == 8 && T == 32) k = MFFT_RK(32, 8);
+  else if (E == 8 && T == 64) k = MFFT_RK(64, 8);
   else if (E == 8 && T == 128) k = MFFT_RK(128, 8);
   else if (E == 8 && T == 256) k = MFFT_RK(256, 8);
   else if (E == 8 && T == 512) k = MFFT_RK(512, 8);
   else if (E == 8 && T == 1024) k = MFFT_RK(1024, 8);
-  else if (E == 16 && T == 64) k = MFFT_RK(64, 16);
-  else if (E == 16 && T == 256) k = MFFT_RK(256, 16);
   else if (E == 16 && T == 1024) k = MFFT_RK(1024, 16);
   else return hipErrorInvalidValue;
 #undef MFFT_RK
@@ -251,7 +264,8 @@ hipError_t launch_reg(bool fwd, const StageArgs &a, const double2 *in, double2 *
       (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytesMax);
   }
   void *args[] = {(void *)&in, (void *)&out, (void *)&tw, (void *)&a};
-  (void)hipLaunchKernel(k, dim3((unsigned)a.nsub), dim3(T), args, lds, stream);
+  const int64_t nblk = (a.nsub + G - 1) / G;
+  (void)hipLaunchKernel(k, dim3((unsigned)nblk), dim3(BT), args, lds, stream);
   return hipGetLastError();
 }
 

@@ -127,6 +127,28 @@ def test_partials_cover_two_stage_plans(lf, oracle):
     assert abs(parts.sum().item() - ref) <= 1e-12 * ref
 
 
+@pytest.mark.parametrize("N,B", [(32, 70), (512, 7), (1024, 3), (8192, 2), (16384, 3), (2100, 5)])
+def test_partials_every_kernel_shape(lf, oracle, N, B):
+    """Weighted transform with sum-of-squares partials on the register-resident kernel (several
+    transforms per workgroup with a partial last workgroup, one per workgroup, 16 elements per
+    thread) and on a large LDS tile: outputs bit-exact, partials sum to sum |out|^2."""
+    x = oracle.lcg_input(N, seed=63 + N, howmany=B)
+    w = np.random.default_rng(N).uniform(0, 1, N)
+    p = lf.Plan(N, -1, B)
+    xd = torch.from_numpy(x).to(DEV)
+    wd = torch.from_numpy(w).to(DEV)
+    y = torch.empty_like(xd)
+    parts = torch.full((p.partials,), float("nan"), dtype=torch.float64, device=DEV)
+    p.call_weighted_sumsq(xd, y, parts, pre=wd, post=wd)
+    torch.cuda.synchronize()
+    ref = oracle.ref_transform(True, p.radices, x * w) * w
+    got = y.cpu().numpy()
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+    assert not torch.isnan(parts).any()
+    tot = float(np.sum(ref.real ** 2 + ref.imag ** 2))
+    assert abs(parts.sum().item() - tot) <= 1e-12 * tot
+
+
 @pytest.mark.skip(reason="host segfault inside torch's CUDAGraph.capture_end when the autograd "
                          "engine's backward runs under stream capture; cause not identified, "
                          "case kept off the GPU (see the design notes)")
