@@ -1,8 +1,9 @@
-// Register-resident Stockham kernel for gfx950: power-of-two transforms of N points, one per
-// transform, T = N / E threads, E elements per thread (8, or 16 at N = 16384) held in
-// registers for the whole transform. Used for N = 64 ... 16384: up to 8192 points it beats
-// the LDS-tile kernel, and at 16384 a transform (256 KiB, more than the 160 KiB of LDS)
-// still fits the registers of one CU, so it runs in one HBM round trip.
+// Register-resident Stockham kernel for gfx950: power-of-two transforms of N points with
+// T = N / E threads per transform, each holding E elements (8, or 16 at N = 16384) in
+// registers for the whole transform; a workgroup runs 256 / T transforms when T < 256, else
+// one. Used for N = 64 ... 16384: up to 8192 points it beats the LDS-tile kernel, and at
+// 16384 a transform (256 KiB, more than the 160 KiB of LDS) still fits the registers of one
+// CU, so it runs in one HBM round trip.
 //
 // Every pass is the reference's passWith (LeanFFTW/Gen/Stockham.lean:88-98) with the
 // butterflies of csrc/bfly_gen.h and the plan's pass tables, so the result is bit-identical
