@@ -83,7 +83,6 @@ __device__ __forceinline__ void reg_exchange(double2 (&v)[E], double *lds, int t
   __syncthreads();
 }
 
-// T = N / E threads; registers held to 128 per thread so that 16 waves fit on a CU.
 // The same exchange with whole complex values through N padded double2 (two barriers).
 template <int R, int T, int E>
 __device__ __forceinline__ void reg_exchange_full(double2 (&v)[E], double2 *lds, int t, int ns) {
@@ -126,6 +125,7 @@ constexpr int reg_block() {
   return T < 256 ? 256 : T;
 }
 
+// Registers are held to 128 per thread so that 16 waves fit on a CU.
 template <bool FWD, int T, int E, bool SQ>
 __global__ __launch_bounds__(reg_block<T>(), 4) void reg_kernel(const double2 *in, double2 *out,
                                                                const double2 *__restrict__ tw,
