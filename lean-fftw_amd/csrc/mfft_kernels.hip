@@ -52,6 +52,7 @@ const void *kernel_for(int fwd, int set, int ept, int sq, int big) {
     return MFFT_DIR(RSET_ALL, 16, 512);
   }
   if (set == RSET_SMALL) return big ? MFFT_DIR(RSET_SMALL, 8, 1024) : MFFT_DIR(RSET_SMALL, 8, 256);
+  if (set == RSET_MID) return big ? MFFT_DIR(RSET_MID, 8, 1024) : MFFT_DIR(RSET_MID, 8, 256);
   return MFFT_DIR(RSET_ALL, 8, 256);
 #undef MFFT_DIR
 #undef MFFT_K
@@ -65,17 +66,21 @@ hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const doub
   const size_t lds = (size_t)stage_lds_elems(a.G, a.L, a.npass, a.stage_io) * sizeof(double2);
   const dim3 grid((unsigned)nx, (unsigned)ntrans);
   // variants: <direction, radix set, elements per thread, max threads, sum of squares>
-  bool small = true, pow2 = true;
+  bool small = true, mid = true, pow2 = true;
   for (int p = 0; p < a.npass; ++p) {
     const int r = a.pass[p].R;
     small = small && small_radix(r);
+    mid = mid && mid_radix(r);
     pow2 = pow2 && (r == 2 || r == 4 || r == 8 || r == 16);
   }
-  const int set = (a.ept == 16 && pow2) ? RSET_POW2 : (small ? RSET_SMALL : RSET_ALL);
+  const int set = (a.ept == 16 && pow2) ? RSET_POW2
+                  : small               ? RSET_SMALL
+                  : (mid && a.ept == 8) ? RSET_MID
+                                        : RSET_ALL;
   const void *k = kernel_for(fwd, set, a.ept, a.sumsq != nullptr, a.threads > 256);
   static const bool attr = [] {
     for (int f = 0; f < 2; ++f)
-      for (int st = 0; st < 3; ++st)
+      for (int st = 0; st < 4; ++st)
         for (int e = 8; e <= 16; e += 8)
           for (int q = 0; q < 2; ++q)
             for (int b = 0; b < 2; ++b)

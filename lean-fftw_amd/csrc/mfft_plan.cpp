@@ -57,12 +57,14 @@ std::atomic<size_t> g_threads{1};
 std::mutex g_wisdom_mu;
 std::string g_wisdom_path;
 
-// Bytes of intermediate per chunk of a two-stage plan: small enough that the chunk's
-// intermediate stays in the 256 MB Infinity Cache between the two stages.
+// Bytes of intermediate per chunk of a two-stage plan (MFFT_CHUNK_MB, default 1 GiB): the
+// bound on the plan's scratch allocation. Smaller chunks do not keep the intermediate in the
+// 256 MB Infinity Cache between the stages on MI355X (16 to 128 MB chunks of a 256 MB batch
+// measured slower than one chunk), so a batch runs as one chunk when it fits.
 int64_t chunk_bytes() {
   static const int64_t v = [] {
     const char *e = std::getenv("MFFT_CHUNK_MB");
-    const int64_t mb = e ? std::atoll(e) : 256;
+    const int64_t mb = e ? std::atoll(e) : 1024;
     return std::max<int64_t>(1, mb) << 20;
   }();
   return v;
@@ -134,21 +136,30 @@ bool best_radices(int64_t L, bool (*ok)(int), std::vector<int> &out, double &cos
 bool any_radix(int) { return true; }
 bool small_radix_ok(int r) { return mfft::small_radix(r); }
 
-// Radix sequence for an LDS stage of length L. A stage whose radices are all small
-// (2, 3, 4, 5, 6, 8) runs in the small-radix kernel variant at higher occupancy (about 88
-// VGPRs against 134), which measured faster than saving a pass with radix 16 (512 points:
-// 5.2 TB/s as 8x8x8 against 3.9 as 16x16x2); so the unrestricted sequence is charged 600
-// per pass for its register cost when it uses a larger radix.
+bool mid_radix_ok(int r) { return mfft::mid_radix(r); }
+
+// Radix sequence for an LDS stage of length L. Stages run the kernel variant of the smallest
+// radix set that holds their radices (small: 2, 3, 4, 5, 6, 8; mid: 2 ... 10; all), whose
+// occupancy is higher the smaller the set (about 88, 100 and 134 VGPRs). The small set
+// measured faster than saving a pass with radix 16 (512 points: 5.2 TB/s as 8x8x8 against
+// 3.9 as 16x16x2), so a sequence is charged 300 per pass when it needs the mid set and 600
+// per pass when it needs the full set, against 1000 per pass.
 std::vector<int> choose_radices(int64_t L) {
-  std::vector<int> all, small;
-  double c_all = 1e300, c_small = 1e300;
-  best_radices(L, any_radix, all, c_all);
-  if (best_radices(L, small_radix_ok, small, c_small)) {
-    bool all_is_small = true;
-    for (int r : all) all_is_small = all_is_small && mfft::small_radix(r);
-    if (!all_is_small) c_all += 600.0 * all.size();
-    if (c_small <= c_all) return small;
-  }
+  std::vector<int> all, mid, small;
+  double c_all = 1e300, c_mid = 1e300, c_small = 1e300;
+  auto charge = [](const std::vector<int> &rs, double &c) {
+    bool s = true, m = true;
+    for (int r : rs) {
+      s = s && mfft::small_radix(r);
+      m = m && mfft::mid_radix(r);
+    }
+    c += (s ? 0.0 : m ? 300.0 : 600.0) * rs.size();
+  };
+  if (best_radices(L, any_radix, all, c_all)) charge(all, c_all);
+  if (best_radices(L, mid_radix_ok, mid, c_mid)) charge(mid, c_mid);
+  if (best_radices(L, small_radix_ok, small, c_small)) charge(small, c_small);
+  if (c_small <= c_mid && c_small <= c_all) return small;
+  if (c_mid <= c_all) return mid;
   return all;
 }
 
@@ -243,6 +254,12 @@ bool all_small(const std::vector<int> &rs) {
   return true;
 }
 
+bool all_mid(const std::vector<int> &rs) {
+  for (int r : rs)
+    if (!mfft::mid_radix(r)) return false;
+  return true;
+}
+
 bool all_pow2(const std::vector<int> &rs) {
   for (int r : rs)
     if (r != 2 && r != 4 && r != 8 && r != 16) return false;
@@ -251,7 +268,7 @@ bool all_pow2(const std::vector<int> &rs) {
 
 // Workgroup shape of a stage whose G, L and radices `rs` are set: 256 threads with 8
 // elements each; or, for a large tile (G * L > LDS_ELEMS), 8 elements per thread and up to
-// 1024 threads when every radix is small, else 16 elements per thread and up to 512 threads.
+// 1024 threads when every radix is at most 10, else 16 elements per thread and up to 512.
 // Thread counts are multiples of 64.
 void set_tile(StageArgs &a, bool big, const std::vector<int> &rs) {
   if (!big) {
@@ -259,7 +276,7 @@ void set_tile(StageArgs &a, bool big, const std::vector<int> &rs) {
     a.threads = mfft::WG_THREADS;
     return;
   }
-  a.ept = all_small(rs) ? 8 : 16;
+  a.ept = all_mid(rs) ? 8 : 16;
   const int64_t elems = (int64_t)a.G * a.L;
   const int64_t per = a.ept * 64;
   a.threads = (int)std::min<int64_t>(a.ept == 8 ? 1024 : 512, ((elems + per - 1) / per) * 64);
@@ -388,8 +405,12 @@ bool build_stages(mfft_plan *p, int split) {
   p->stages.push_back(B);
   p->stage_passes.push_back(split);
   p->stage_passes.push_back(s - split);
-  p->chunk_h = std::max<int64_t>(1, std::min<int64_t>(p->howmany, chunk_bytes() / (16 * N)));
-  p->chunk_h = std::min<int64_t>(p->chunk_h, 65535);  // grid.y
+  // equal chunks: a short last chunk is a latency-bound launch pair of its own (10 to 25 us
+  // at 10368..165375 points)
+  int64_t cmax = std::max<int64_t>(1, std::min<int64_t>(p->howmany, chunk_bytes() / (16 * N)));
+  cmax = std::min<int64_t>(cmax, 65535);  // grid.y
+  const int64_t nchunk = (std::max<int64_t>(1, p->howmany) + cmax - 1) / cmax;
+  p->chunk_h = (std::max<int64_t>(1, p->howmany) + nchunk - 1) / nchunk;
   return true;
 }
 
@@ -479,6 +500,7 @@ std::vector<std::vector<int>> seq_candidates(int64_t L) {
     std::vector<int> v;
     double c;
     if (best_radices(L, any_radix, v, c, cap)) add(v);
+    if (best_radices(L, mid_radix_ok, v, c, cap)) add(v);
     if (best_radices(L, small_radix_ok, v, c, cap)) add(v);
   }
   return out;
