@@ -17,15 +17,16 @@ void cos_sin_2pi_table(uint64_t n, std::vector<double> &cs);  // 2n doubles: c0,
 
 // ---------------------------------------------------------------- schedule
 // Radix sets of the kernel variants. A kernel's register allocation is set by the largest
-// butterfly it contains, so stages run the smallest set holding their radices: SMALL
-// (2, 3, 4, 5, 6, 8: about 88 VGPRs at 8 elements per thread), MID (2 ... 10: about 100, four
-// waves per SIMD), ALL (every radix up to 16: about 134, three waves); POW2 is the set of the
-// large-tile variant with 16 elements per thread.
+// butterflies it contains, so stages run the smallest set holding their radices: SMALL
+// (2, 3, 4, 5, 6, 8: about 88 VGPRs at 8 elements per thread), MID (2 ... 13, enough for
+// every 13-smooth size: about 110, four waves per SIMD), ALL (up to 16, with 14, 15 and 16:
+// about 134, three waves); POW2 is the set of the large-tile variant with 16 elements per
+// thread.
 enum RadixSet { RSET_ALL = 0, RSET_SMALL = 1, RSET_POW2 = 2, RSET_MID = 3 };
 __host__ __device__ constexpr bool small_radix(int r) {
   return r == 2 || r == 3 || r == 4 || r == 5 || r == 6 || r == 8;
 }
-__host__ __device__ constexpr bool mid_radix(int r) { return r >= 2 && r <= 10; }
+__host__ __device__ constexpr bool mid_radix(int r) { return r >= 2 && r <= 13; }
 __host__ __device__ constexpr bool in_set(int set, int r) {
   return set == RSET_ALL || (set == RSET_SMALL ? small_radix(r)
                              : set == RSET_MID ? mid_radix(r)

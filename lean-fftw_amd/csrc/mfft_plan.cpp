@@ -139,12 +139,13 @@ bool small_radix_ok(int r) { return mfft::small_radix(r); }
 bool mid_radix_ok(int r) { return mfft::mid_radix(r); }
 
 // Radix sequence for an LDS stage of length L. Stages run the kernel variant of the smallest
-// radix set that holds their radices (small: 2, 3, 4, 5, 6, 8; mid: 2 ... 10; all), whose
-// occupancy is higher the smaller the set (about 88, 100 and 134 VGPRs). The small set
+// radix set that holds their radices (small: 2, 3, 4, 5, 6, 8; mid: 2 ... 13; all), whose
+// occupancy is higher the smaller the set (about 88, 110 and 134 VGPRs). The small set
 // measured faster than saving a pass with radix 16 (512 points: 5.2 TB/s as 8x8x8 against
 // 3.9 as 16x16x2), so a sequence is charged 300 per pass when it needs the mid set and 600
-// per pass when it needs the full set, against 1000 per pass.
-std::vector<int> choose_radices(int64_t L) {
+// per pass when it needs the full set, against 1000 per pass. `mid_only` (ragged batches,
+// whose kernel holds the mid set) leaves out radices 14, 15 and 16.
+std::vector<int> choose_radices(int64_t L, bool mid_only = false) {
   std::vector<int> all, mid, small;
   double c_all = 1e300, c_mid = 1e300, c_small = 1e300;
   auto charge = [](const std::vector<int> &rs, double &c) {
@@ -155,7 +156,7 @@ std::vector<int> choose_radices(int64_t L) {
     }
     c += (s ? 0.0 : m ? 300.0 : 600.0) * rs.size();
   };
-  if (best_radices(L, any_radix, all, c_all)) charge(all, c_all);
+  if (!mid_only && best_radices(L, any_radix, all, c_all)) charge(all, c_all);
   if (best_radices(L, mid_radix_ok, mid, c_mid)) charge(mid, c_mid);
   if (best_radices(L, small_radix_ok, small, c_small)) charge(small, c_small);
   if (c_small <= c_mid && c_small <= c_all) return small;
@@ -645,6 +646,7 @@ bool measure(const mfft_plan *shape, Cand &best) {
   (void)hipEventCreate(&e1);
   double best_us = 1e300;
   for (const Cand &c : schedule_candidates(n, shape->allow_big)) {
+    if (!shape->allow_big && !all_mid(c.rs)) continue;  // ragged classes: the mid kernel
     mfft_plan *q = clone_shape(shape, hm);
     q->radices = c.rs;
     q = finish(q, c.split);
@@ -707,14 +709,14 @@ mfft_plan *plan_many_internal(int64_t n, int sign, int64_t howmany, int64_t istr
     }
   }
   if (single_stage(n, allow_big)) {
-    p->radices = allow_big && reg_size(n) ? reg_radices(n) : choose_radices(n);
+    p->radices = allow_big && reg_size(n) ? reg_radices(n) : choose_radices(n, !allow_big);
   } else {
     int64_t P;
     if (!choose_split(n, P)) {
       delete p;
       return nullptr;
     }
-    std::vector<int> a = choose_radices(P), b = choose_radices(n / P);
+    std::vector<int> a = choose_radices(P, !allow_big), b = choose_radices(n / P, !allow_big);
     p->radices = a;
     p->radices.insert(p->radices.end(), b.begin(), b.end());
     split = (int)a.size();
@@ -726,6 +728,7 @@ int mfft::export_class(const mfft_plan *p, mfft::RClass *c, std::vector<double2>
   if (!p || p->howmany != 1 || p->stages.empty() || p->stages.size() > 2) return 1;
   for (const auto &st : p->stages)
     if (st.ept != 8) return 1;  // the ragged kernel runs the 256-thread layout only
+  if (!all_mid(p->radices)) return 1;  // and holds the mid radix set
   const int64_t base = (int64_t)tw->size();
   *c = mfft::RClass{};
   c->N = (int)p->n;

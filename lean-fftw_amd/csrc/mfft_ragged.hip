@@ -96,7 +96,7 @@ __global__ __launch_bounds__(WG_THREADS) void ragged_kernel(
       hout = HbmSide{out + t.base + t.sub0, nullptr, 1, c.P, nullptr, 0, 0};
       gin = gout = 1;
     }
-    (void)run_stage<FWD, RSET_ALL, LDS_ELEMS / WG_THREADS>(lds, hin, hout, tid, WG_THREADS, c.G[st],
+    (void)run_stage<FWD, RSET_MID, LDS_ELEMS / WG_THREADS>(lds, hin, hout, tid, WG_THREADS, c.G[st],
                                                      gcount, L, c.npass[st], c.pass[st], gin, gout,
                                                      tw, t.sub0, st == 1 ? 1 : 0);
     if (c.nstage == 2 && st == 0 && wait) {
