@@ -164,12 +164,14 @@ std::vector<int> choose_radices(int64_t L, bool mid_only = false) {
   return all;
 }
 
-// Radix sequence of a register-resident size: radices up to 8 with 8 elements per thread
-// (N <= 8192), radix 4 only with 16 (N = 16384).
+// Radix sequence of a register-resident size: radices up to 8 (the fixed schedules of
+// mfft_reg.hip for 2048 ... 8192), and 16 x 16 x 16 x 4 at 16384 (a fixed schedule with 16
+// elements per thread: 125 us per 256 MB batch against 180 for radix 4 only).
 std::vector<int> reg_radices(int64_t n) {
+  if (n == 16384) return {16, 16, 16, 4};
   std::vector<int> rs;
   double c;
-  best_radices(n, any_radix, rs, c, mfft::reg_elems(n) == 16 ? 4 : 8);
+  best_radices(n, any_radix, rs, c, 8);
   return rs;
 }
 

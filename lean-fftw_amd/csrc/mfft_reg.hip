@@ -125,7 +125,65 @@ constexpr int reg_block() {
   return T < 256 ? 256 : T;
 }
 
-// Registers are held to 128 per thread so that 16 waves fit on a CU.
+// Load a transform's columns t + T m (non-temporal), times the input weights.
+template <int T, int E>
+__device__ __forceinline__ void reg_load(double2 (&v)[E], const StageArgs &a, const double2 *src,
+                                         int t, bool valid) {
+#pragma unroll
+  for (int m = 0; m < E; ++m)
+    v[m] = valid ? ld_stream(src + (int64_t)(t + T * m) * a.in_e) : make_double2(0.0, 0.0);
+  if (a.pre) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const double w = a.pre[(int64_t)(t + T * m) * a.pre_e];
+      v[m].x = v[m].x * w;
+      v[m].y = v[m].y * w;
+    }
+  }
+}
+
+// Store after the last pass (ns = N / R): output r of butterfly j = t + T u goes to
+// j + r N/R, the column of the register that holds it, so register m goes to element t + T m.
+// Returns this thread's sum of |out|^2. (The opaque copy of t keeps the compiler from
+// computing these addresses before the passes and holding them in registers through them.)
+template <int T, int E>
+__device__ __forceinline__ double reg_store(const double2 (&v)[E], const StageArgs &a,
+                                            double2 *dst, int t, bool valid) {
+  int ts = t;
+  asm volatile("" : "+v"(ts));
+  double sumsq = 0.0;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int e = ts + T * m;
+    double2 y = v[m];
+    if (a.post) {
+      const double w = a.post[(int64_t)e * a.post_e];
+      y.x = y.x * w;
+      y.y = y.y * w;
+    }
+    if (valid) st_stream(dst + (int64_t)e * a.out_e, y);
+    sumsq = sumsq + (y.x * y.x + y.y * y.y);
+  }
+  return sumsq;
+}
+
+// One sum-of-squares partial per workgroup, a fixed reduction tree.
+template <int BT>
+__device__ __forceinline__ void block_partial(double sumsq, double *dst) {
+  __shared__ double wsum[BT / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) sumsq = sumsq + __shfl_xor(sumsq, o, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = sumsq;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < BT / 64; ++w) s = s + wsum[w];
+    *dst = s;
+  }
+}
+
+// Any radix sequence in {2, 4, 8} (16 elements per thread: radix 4 only), dispatched pass by
+// pass at run time. Registers are held to 128 per thread so that 16 waves fit on a CU.
 template <bool FWD, int T, int E, bool SQ>
 __global__ __launch_bounds__(reg_block<T>(), 4) void reg_kernel(const double2 *in, double2 *out,
                                                                const double2 *__restrict__ tw,
@@ -140,20 +198,8 @@ __global__ __launch_bounds__(reg_block<T>(), 4) void reg_kernel(const double2 *i
   void *lds_d = lds_c + g * lds_len(T * E);
   const int64_t h = (int64_t)blockIdx.x * G + g;
   const bool valid = h < a.nsub;
-  const double2 *src = in + (valid ? h : 0) * a.in_g;
-  double2 *dst = out + (valid ? h : 0) * a.out_g;
   double2 v[E];
-#pragma unroll
-  for (int m = 0; m < E; ++m)
-    v[m] = valid ? ld_stream(src + (int64_t)(t + T * m) * a.in_e) : make_double2(0.0, 0.0);
-  if (a.pre) {
-#pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const double w = a.pre[(int64_t)(t + T * m) * a.pre_e];
-      v[m].x = v[m].x * w;
-      v[m].y = v[m].y * w;
-    }
-  }
+  reg_load<T, E>(v, a, in + (valid ? h : 0) * a.in_g, t, valid);
   for (int p = 0; p < a.npass; ++p) {
     const PassArgs pa = a.pass[p];
     const bool twd = pa.twoff >= 0;
@@ -185,38 +231,75 @@ __global__ __launch_bounds__(reg_block<T>(), 4) void reg_kernel(const double2 *i
     }
 #undef MFFT_REG
   }
-  // the last pass (ns = N / R) writes output r of butterfly j = t + T u to j + r N/R, the
-  // column of the register that holds it: register m goes to element t + T m. (The opaque
-  // copy of t keeps the compiler from computing these addresses before the pass loop and
-  // holding them in registers through it.)
-  int ts = t;
-  asm volatile("" : "+v"(ts));
-  double sumsq = 0.0;
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const int e = ts + T * m;
-    double2 y = v[m];
-    if (a.post) {
-      const double w = a.post[(int64_t)e * a.post_e];
-      y.x = y.x * w;
-      y.y = y.y * w;
-    }
-    if (valid) st_stream(dst + (int64_t)e * a.out_e, y);
-    if constexpr (SQ) sumsq = sumsq + (y.x * y.x + y.y * y.y);
+  const double sumsq = reg_store<T, E>(v, a, out + (valid ? h : 0) * a.out_g, t, valid);
+  if constexpr (SQ) block_partial<reg_block<T>()>(sumsq, a.sumsq + blockIdx.x);
+}
+
+// ---------------------------------------------------------------- fixed schedules
+// The default schedules of 2048 ... 16384 points as compile-time radix sequences: ns, the
+// twiddle flag and the exchange offsets become constants and the passes straight-line code,
+// which takes the kernel from about 104 to about 61 registers at 8 elements per thread (and
+// lets 16 elements per thread run radix 8 and 16 without spilling). With the exchange of
+// real then imaginary parts (N doubles of LDS) two workgroups of 1024 threads share a CU at
+// 8192 points, four of 512 at 4096.
+template <bool FWD, int T, int E, bool HALF, int P, int NS, int R, int... Rest>
+__device__ __forceinline__ void fixed_passes(double2 (&v)[E], int t, void *lds,
+                                             const StageArgs &a, const double2 *__restrict__ tw) {
+  reg_bfly<FWD, R, T, E>(v, t, NS, tw + (P > 0 ? a.pass[P].twoff : 0), P > 0);
+  if constexpr (sizeof...(Rest) > 0) {
+    if constexpr (HALF)
+      reg_exchange<R, T, E>(v, static_cast<double *>(lds), t, NS);
+    else
+      reg_exchange_full<R, T, E>(v, static_cast<double2 *>(lds), t, NS);
+    fixed_passes<FWD, T, E, HALF, P + 1, NS * R, Rest...>(v, t, lds, a, tw);
   }
-  if constexpr (SQ) {  // one partial per workgroup, a fixed reduction tree
-    constexpr int BT = reg_block<T>();
-    __shared__ double wsum[BT / 64];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) sumsq = sumsq + __shfl_xor(sumsq, o, 64);
-    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = sumsq;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int w = 0; w < BT / 64; ++w) s = s + wsum[w];
-      a.sumsq[blockIdx.x] = s;
-    }
+}
+
+template <bool FWD, int T, int E, bool SQ, int... Rs>
+__global__ __launch_bounds__(T, 4) void fixed_kernel(const double2 *in, double2 *out,
+                                                    const double2 *__restrict__ tw,
+                                                    const StageArgs a) {
+  extern __shared__ double2 lds_c[];
+  const int t = threadIdx.x;
+  const int64_t h = blockIdx.x;
+  double2 v[E];
+  reg_load<T, E>(v, a, in + h * a.in_g, t, true);
+  fixed_passes<FWD, T, E, true, 0, 1, Rs...>(v, t, lds_c, a, tw);
+  const double sumsq = reg_store<T, E>(v, a, out + h * a.out_g, t, true);
+  if constexpr (SQ) block_partial<T>(sumsq, a.sumsq + blockIdx.x);
+}
+
+// The fixed schedules: N, elements per thread, radix sequence, and the kernels by
+// (direction, sum of squares).
+struct FixedSchedule {
+  int n, e, npass;
+  int rs[8];
+  const void *k[2][2];
+};
+
+#define MFFT_FX(TT, EE, ...)                                                                 \
+  {{(const void *)fixed_kernel<false, TT, EE, false, __VA_ARGS__>,                            \
+    (const void *)fixed_kernel<false, TT, EE, true, __VA_ARGS__>},                            \
+   {(const void *)fixed_kernel<true, TT, EE, false, __VA_ARGS__>,                             \
+    (const void *)fixed_kernel<true, TT, EE, true, __VA_ARGS__>}}
+const FixedSchedule kFixed[] = {
+    {2048, 8, 4, {8, 8, 8, 4}, MFFT_FX(256, 8, 8, 8, 8, 4)},
+    {4096, 8, 4, {8, 8, 8, 8}, MFFT_FX(512, 8, 8, 8, 8, 8)},
+    {8192, 8, 5, {8, 8, 8, 8, 2}, MFFT_FX(1024, 8, 8, 8, 8, 8, 2)},
+    {8192, 16, 6, {8, 4, 4, 4, 4, 4}, MFFT_FX(512, 16, 8, 4, 4, 4, 4, 4)},
+    {16384, 16, 4, {16, 16, 16, 4}, MFFT_FX(1024, 16, 16, 16, 16, 4)},
+    {16384, 16, 5, {8, 8, 8, 8, 4}, MFFT_FX(1024, 16, 8, 8, 8, 8, 4)},
+};
+#undef MFFT_FX
+
+const FixedSchedule *fixed_schedule(int64_t n, const int *radices, int nradix) {
+  for (const auto &f : kFixed) {
+    if (f.n != n || f.npass != nradix) continue;
+    bool same = true;
+    for (int i = 0; i < nradix; ++i) same = same && f.rs[i] == radices[i];
+    if (same) return &f;
   }
+  return nullptr;
 }
 
 // Elements per thread: 8 (T = N / 8 <= 1024 threads) up to 8192 points, 16 at 16384.
@@ -224,6 +307,7 @@ int reg_elems(int64_t n) { return n > 8192 ? 16 : 8; }
 
 bool reg_supported(int64_t n, const int *radices, int nradix) {
   if (n < 32 || n > 16384 || (n & (n - 1)) || nradix < 1 || nradix > MAX_PASSES) return false;
+  if (fixed_schedule(n, radices, nradix)) return true;
   for (int i = 0; i < nradix; ++i) {
     const int r = radices[i];
     if (reg_elems(n) == 16 ? r != 4 : (r != 2 && r != 4 && r != 8)) return false;
@@ -234,6 +318,23 @@ bool reg_supported(int64_t n, const int *radices, int nradix) {
 hipError_t launch_reg(bool fwd, const StageArgs &a, const double2 *in, double2 *out,
                       const double2 *tw, hipStream_t stream) {
   if (a.nsub <= 0) return hipSuccess;
+  int rs[MAX_PASSES];
+  for (int p = 0; p < a.npass; ++p) rs[p] = a.pass[p].R;
+  if (const FixedSchedule *f = fixed_schedule(a.L, rs, a.npass)) {
+    const void *k = f->k[fwd ? 1 : 0][a.sumsq ? 1 : 0];
+    const int T = a.L / f->e;
+    const size_t lds = (size_t)reg_lds_doubles(a.L) * sizeof(double);
+    {
+      static std::mutex mu;
+      static std::set<const void *> done;
+      std::lock_guard<std::mutex> lk(mu);
+      if (done.insert(k).second)
+        (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytesMax);
+    }
+    void *args[] = {(void *)&in, (void *)&out, (void *)&tw, (void *)&a};
+    (void)hipLaunchKernel(k, dim3((unsigned)a.nsub), dim3(T), args, lds, stream);
+    return hipGetLastError();
+  }
   const int E = reg_elems(a.L), T = a.L / E;
   const int BT = T < 256 ? 256 : T, G = BT / T;  // as reg_block<T>()
   const size_t lds = E == 8 ? (size_t)G * lds_len(a.L) * sizeof(double2)

@@ -98,6 +98,21 @@ def test_stage_splits_bit_exact(lf, oracle, N, rs, split):
     assert bits_equal(got, oracle.ref_transform(True, rs, x))
 
 
+@pytest.mark.parametrize("N,rs", [(2048, [8, 8, 8, 4]), (4096, [8, 8, 8, 8]), (8192, [8, 8, 8, 8, 2]),
+                                  (8192, [8, 4, 4, 4, 4, 4]), (16384, [16, 16, 16, 4]),
+                                  (16384, [8, 8, 8, 8, 4]), (16384, [4, 4, 4, 4, 4, 4, 4]),
+                                  (4096, [4, 8, 2, 8, 8]), (1024, [2, 4, 8, 4, 4])])
+def test_register_schedules_bit_exact(lf, oracle, N, rs):
+    """The register-resident kernel's fixed (compile-time) schedules and its run-time
+    dispatched ones, both directions, in and out of place."""
+    x = oracle.lcg_input(N, seed=31 + N + len(rs), howmany=3)
+    for fwd in (True, False):
+        plan = lf.Plan(N, lf.FORWARD if fwd else lf.BACKWARD, 3, radices=rs)
+        ref = oracle.ref_transform(fwd, rs, x)
+        assert bits_equal(run(lf, plan, x), ref), (N, rs, fwd)
+        assert bits_equal(run(lf, plan, x, inplace=True), ref), (N, rs, fwd, "in place")
+
+
 def test_inplace_and_batch_tail(lf, oracle):
     for N, howmany in ((1024, 7), (60, 1001), (8192, 5), (2, 3333)):
         x = oracle.lcg_input(N, seed=5 + N, howmany=howmany)
