@@ -74,7 +74,9 @@ struct StageArgs {
   int gfast_in, gfast_out;  // 1: consecutive threads walk g (g adjacent in memory)
   int stage_io;             // 1: coalesced tile copies HBM -> LDS and back (contiguous small L)
   int reg;                  // 1: register-resident kernel (mfft_reg.hip), one transform of
-                            // L = 32..16384 points, L / 8 or L / 16 threads per transform
+                            // L = 32..16384 points, L / 8 or L / 16 threads per transform;
+                            // 2: register-resident stage kernel, G sub-problems of L / 8
+                            // threads each
   uint32_t m_L;             // ceil(2^32 / L), for the tile copies
   PassArgs pass[MAX_PASSES];
   int64_t nsub;
@@ -103,6 +105,13 @@ int export_class(const struct ::mfft_plan *p, RClass *c, std::vector<double2> *t
 // The register-resident kernel: power-of-two n in [32, 8192] with radices in {2, 4, 8}, or
 // n = 16384 as radix 4 only.
 bool reg_supported(int64_t n, const int *radices, int nradix);
+// Register-resident stage kernels (mfft_reg.hip): the sub-problems per workgroup of the
+// stage schedule for length L and these radices (the preferred one if its G <= gmax, else the
+// largest G <= gmax; 0 if there is none); and the launch of a stage whose G is one of them
+// (sum-of-squares stages run the stage kernel).
+int reg_stage_g(int L, const int *radices, int nradix, int gmax);
+hipError_t launch_reg_stage(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,
+                            double2 *out, const double2 *tw, hipStream_t stream);
 int reg_elems(int64_t n);  // elements per thread of the register-resident kernel at size n
 hipError_t launch_reg(bool fwd, const StageArgs &a, const double2 *in, double2 *out,
                       const double2 *tw, hipStream_t stream);

@@ -61,7 +61,8 @@ const void *kernel_for(int fwd, int set, int ept, int sq, int big) {
 hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,
                         double2 *out, const double2 *tw, hipStream_t stream) {
   if (a.nsub <= 0 || ntrans <= 0) return hipSuccess;
-  if (a.reg) return ntrans == 1 ? launch_reg(fwd, a, in, out, tw, stream) : hipErrorInvalidValue;
+  if (a.reg == 1) return ntrans == 1 ? launch_reg(fwd, a, in, out, tw, stream) : hipErrorInvalidValue;
+  if (a.reg == 2 && !a.sumsq) return launch_reg_stage(fwd, a, ntrans, in, out, tw, stream);
   const int64_t nx = (a.nsub + a.G - 1) / a.G;
   const size_t lds = (size_t)stage_lds_elems(a.G, a.L, a.npass, a.stage_io) * sizeof(double2);
   const dim3 grid((unsigned)nx, (unsigned)ntrans);

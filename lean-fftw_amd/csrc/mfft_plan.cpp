@@ -300,6 +300,24 @@ void strided_tile(StageArgs &a, int64_t L, int64_t U, bool allow_big, const std:
   set_tile(a, big && a.G * L > mfft::LDS_ELEMS, rs);
 }
 
+// A strided stage whose length and radices have a register-resident stage schedule
+// (mfft_reg.hip) runs it, with the schedule's preferred number of sub-problems per workgroup
+// (MFFT_RSTAGE_G caps it; 0 turns these kernels off); the tile shape also suits the stage
+// kernel, which runs the stage when sum-of-squares partials are requested.
+void reg_stage(StageArgs &a, const std::vector<int> &rs, int64_t U) {
+  static const int gmax = [] {
+    const char *e = std::getenv("MFFT_RSTAGE_G");
+    return e ? std::atoi(e) : 1 << 20;
+  }();
+  if (gmax <= 0 || !all_small(rs)) return;
+  const int G = mfft::reg_stage_g(a.L, rs.data(), (int)rs.size(), gmax);
+  if (G <= 0 || G > U) return;
+  a.reg = 2;
+  a.G = G;
+  a.ept = 8;
+  a.threads = a.L / 8 * G;
+}
+
 void fill_pass(mfft::PassArgs &pa, int R, int64_t nsl, int64_t twns, int64_t kmul, int L,
                int G) {
   pa.R = R;
@@ -373,6 +391,7 @@ bool build_stages(mfft_plan *p, int split) {
   A.L = (int)P;
   A.npass = split;
   strided_tile(A, P, Q, p->allow_big, std::vector<int>(rs.begin(), rs.begin() + split));
+  if (p->allow_big) reg_stage(A, std::vector<int>(rs.begin(), rs.begin() + split), Q);
   for (int t = 0; t < split; ++t) fill_pass(A.pass[t], rs[t], ns[t], ns[t], 1, A.L, A.G);
   A.nsub = Q;
   A.in_g = p->istride;  // column c starts at element c
@@ -390,6 +409,7 @@ bool build_stages(mfft_plan *p, int split) {
   B.L = (int)Q;
   B.npass = s - split;
   strided_tile(B, Q, P, p->allow_big, std::vector<int>(rs.begin() + split, rs.end()));
+  if (p->allow_big) reg_stage(B, std::vector<int>(rs.begin() + split, rs.end()), P);
   for (int t = split; t < s; ++t)
     fill_pass(B.pass[t - split], rs[t], ns[t] / P, ns[t], P, B.L, B.G);
   B.nsub = P;  // residue classes k1 + P i

@@ -302,6 +302,213 @@ const FixedSchedule *fixed_schedule(int64_t n, const int *radices, int nradix) {
   return nullptr;
 }
 
+// ---------------------------------------------------------------- register-resident stages
+// A kernel stage of a two-stage plan, N = P * Q, with the same scheme: the workgroup takes G
+// sub-problems of length L (Q decimated columns of length P, or P residue classes of length
+// Q), T = L / E threads each, and holds them in registers through the stage's passes. Loads
+// walk the sub-problem index with consecutive lanes (rows of G elements: 512 bytes at
+// G = 32), and so do the stores of stage B; stage A's stores walk the element index instead
+// (its intermediate is contiguous per column), so the exchange before the last pass
+// redistributes the columns to that order. Each sub-problem has its own LDS region of
+// reg_lds_doubles(L) doubles (real then imaginary parts), so G = 32 sub-problems of 256
+// points take 68 KiB and two workgroups share a CU. Twiddles as in the stage kernel: pass p
+// of sub-problem g0 + g reads tw[twoff + (r - 1) twns + (g0 + g) kb_u + k kmul].
+
+// Twiddled butterflies of a stage pass (k = j mod NS, NS the product of the stage's earlier
+// radices).
+template <bool FWD, int R, int T, int E, int NS>
+__device__ __forceinline__ void stage_bfly(double2 (&v)[E], int t, const double2 *__restrict__ twp,
+                                           bool twd, int64_t twns, int64_t kb, int64_t kmul) {
+  constexpr int NB = E / R;
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    double2 x[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = v[u + r * NB];
+    if (twd) {
+      const int k = (t + u * T) & (NS - 1);
+      const double2 *tp = twp + kb + (int64_t)k * kmul;
+#pragma unroll
+      for (int r = 1; r < R; ++r) x[r] = twmul(tp[(r - 1) * twns], x[r]);
+    }
+    Bfly<FWD, R, false>::run(x);
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[u + r * NB] = x[r];
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Exchange of a stage pass of radix R: the writer (region gw, thread tw) puts output r of
+// butterfly u at its Stockham position; the reader (region gr, thread tr) takes columns
+// tr + T m. Real parts, then imaginary parts.
+template <int R, int T, int E, int NS, int SUB>
+__device__ __forceinline__ void stage_exchange(double2 (&v)[E], double *lds, int gw, int tw,
+                                               int gr, int tr) {
+  constexpr int NB = E / R;
+  double *wr = lds + gw * SUB;
+  int pos[E];
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    const int j = tw + u * T;
+    const int k = j & (NS - 1);
+    const int base = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) pos[u + r * NB] = rpad(base + r * NS);
+  }
+  const double *rd = lds + gr * SUB;
+#pragma unroll
+  for (int i = 0; i < E; ++i) wr[pos[i]] = v[i].x;
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m].x = rd[rpad(tr + T * m)];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < E; ++i) wr[pos[i]] = v[i].y;
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m].y = rd[rpad(tr + T * m)];
+  __syncthreads();
+}
+
+// The passes of a stage from pass P on: the threads keep the input order (gi, ti) until the
+// exchange before the last pass, which hands them the output order (go, to). kbi / kbo are
+// the twiddle bases of the sub-problems in the two orders (those of a valid sub-problem for
+// the lanes past a partial tile, whose results are not stored, so that every twiddle read
+// stays inside the table).
+template <bool FWD, int T, int E, int SUB, int P, int NS, int R, int... Rest>
+__device__ __forceinline__ void stage_passes(double2 (&v)[E], double *lds, int gi, int ti, int go,
+                                             int to, const StageArgs &a,
+                                             const double2 *__restrict__ tw, int64_t kbi,
+                                             int64_t kbo) {
+  const PassArgs &pa = a.pass[P];
+  const bool twd = P > 0 || pa.twoff >= 0;
+  const int t = sizeof...(Rest) > 0 ? ti : to;
+  const int64_t kb = sizeof...(Rest) > 0 ? kbi : kbo;
+  stage_bfly<FWD, R, T, E, NS>(v, t, tw + (twd ? pa.twoff : 0), twd, pa.twns, kb, pa.kmul);
+  if constexpr (sizeof...(Rest) > 0) {
+    constexpr bool before_last = sizeof...(Rest) == 1;
+    stage_exchange<R, T, E, NS, SUB>(v, lds, gi, ti, before_last ? go : gi, before_last ? to : ti);
+    stage_passes<FWD, T, E, SUB, P + 1, NS * R, Rest...>(v, lds, gi, ti, go, to, a, tw, kbi, kbo);
+  }
+}
+
+template <bool FWD, int L, int E, int G, int... Rs>
+__global__ __launch_bounds__(L / E * G, 4) void stage_reg_kernel(const double2 *in, double2 *out,
+                                                                const double2 *__restrict__ tw,
+                                                                const StageArgs a) {
+  constexpr int T = L / E, SUB = reg_lds_doubles(L);
+  static_assert(sizeof...(Rs) >= 2, "a stage of one pass has no exchange to reorder its threads");
+  extern __shared__ double lds_s[];
+  const int i = threadIdx.x;
+  const int64_t g0 = (int64_t)blockIdx.x * G;
+  const int64_t h = blockIdx.y;
+  const int gcount = (int)(a.nsub - g0 < G ? a.nsub - g0 : G);
+  const int gi = a.gfast_in ? i % G : i / T, ti = a.gfast_in ? i / G : i % T;
+  const int go = a.gfast_out ? i % G : i / T, to = a.gfast_out ? i / G : i % T;
+  double2 v[E];
+  {
+    const bool ok = gi < gcount;
+    const double2 *src = in + h * a.in_h + (g0 + (ok ? gi : 0)) * a.in_g;
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+      v[m] = ok ? ld_stream(src + (int64_t)(ti + T * m) * a.in_e) : make_double2(0.0, 0.0);
+    if (a.pre) {
+      const double *w = a.pre + (g0 + (ok ? gi : 0)) * a.pre_g;
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const double c = w[(int64_t)(ti + T * m) * a.pre_e];
+        v[m].x = v[m].x * c;
+        v[m].y = v[m].y * c;
+      }
+    }
+  }
+  const int64_t kbi = (g0 + (gi < gcount ? gi : 0)) * a.kb_u;
+  const int64_t kbo = (g0 + (go < gcount ? go : 0)) * a.kb_u;
+  stage_passes<FWD, T, E, SUB, 0, 1, Rs...>(v, lds_s, gi, ti, go, to, a, tw, kbi, kbo);
+  int tos = to;
+  asm volatile("" : "+v"(tos));  // keep the store addresses out of the pass code
+  if (go < gcount) {
+    double2 *dst = out + h * a.out_h + (g0 + go) * a.out_g;
+    const double *w = a.post ? a.post + (g0 + go) * a.post_g : nullptr;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int e = tos + T * m;
+      double2 y = v[m];
+      if (w) {
+        const double c = w[(int64_t)e * a.post_e];
+        y.x = y.x * c;
+        y.y = y.y * c;
+      }
+      st_stream(dst + (int64_t)e * a.out_e, y);
+    }
+  }
+}
+
+// The stage schedules: sub-problem length, sub-problems per workgroup, radix sequence, and
+// whether it is the one the planner prefers for that length. Measured on MI355X (256 MB
+// batches; per stage): 128 points 112 / 102 / 104 us at G = 8 / 16 / 32, 256 points
+// 108 / 103 / 118 us at 8 / 16 / 32, 512 points 108 / 120 us at 8 / 16 (the LDS-tile stage
+// kernel: 107, 113, 116 us). Wider rows lose more to occupancy than they gain.
+struct StageSchedule {
+  int L, G, npass;
+  int rs[4];
+  bool pref;
+  const void *k[2];
+};
+
+#define MFFT_SK(LL, GG, ...)                                               \
+  {(const void *)stage_reg_kernel<false, LL, 8, GG, __VA_ARGS__>,         \
+   (const void *)stage_reg_kernel<true, LL, 8, GG, __VA_ARGS__>}
+const StageSchedule kStages[] = {
+    {128, 8, 3, {8, 8, 2}, false, MFFT_SK(128, 8, 8, 8, 2)},
+    {128, 16, 3, {8, 8, 2}, true, MFFT_SK(128, 16, 8, 8, 2)},
+    {128, 32, 3, {8, 8, 2}, false, MFFT_SK(128, 32, 8, 8, 2)},
+    {256, 8, 3, {8, 8, 4}, false, MFFT_SK(256, 8, 8, 8, 4)},
+    {256, 16, 3, {8, 8, 4}, true, MFFT_SK(256, 16, 8, 8, 4)},
+    {256, 32, 3, {8, 8, 4}, false, MFFT_SK(256, 32, 8, 8, 4)},
+    {512, 8, 3, {8, 8, 8}, true, MFFT_SK(512, 8, 8, 8, 8)},
+    {512, 16, 3, {8, 8, 8}, false, MFFT_SK(512, 16, 8, 8, 8)},
+};
+#undef MFFT_SK
+
+int reg_stage_g(int L, const int *radices, int nradix, int gmax) {
+  int best = 0;
+  for (const auto &st : kStages) {
+    if (st.L != L || st.npass != nradix || st.G > gmax) continue;
+    bool same = true;
+    for (int i = 0; i < nradix; ++i) same = same && st.rs[i] == radices[i];
+    if (!same) continue;
+    if (st.pref) return st.G;
+    best = st.G > best ? st.G : best;
+  }
+  return best;
+}
+
+hipError_t launch_reg_stage(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,
+                            double2 *out, const double2 *tw, hipStream_t stream) {
+  const void *k = nullptr;
+  for (const auto &st : kStages) {
+    if (st.L != a.L || st.G != a.G || st.npass != a.npass) continue;
+    bool same = true;
+    for (int i = 0; i < a.npass; ++i) same = same && st.rs[i] == a.pass[i].R;
+    if (same) k = st.k[fwd ? 1 : 0];
+  }
+  if (!k) return hipErrorInvalidValue;
+  {
+    static std::mutex mu;
+    static std::set<const void *> done;
+    std::lock_guard<std::mutex> lk(mu);
+    if (done.insert(k).second)
+      (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytesMax);
+  }
+  const int64_t nx = (a.nsub + a.G - 1) / a.G;
+  const size_t lds = (size_t)a.G * reg_lds_doubles(a.L) * sizeof(double);
+  const int bt = a.L / 8 * a.G;
+  void *args[] = {(void *)&in, (void *)&out, (void *)&tw, (void *)&a};
+  (void)hipLaunchKernel(k, dim3((unsigned)nx, (unsigned)ntrans), dim3(bt), args, lds, stream);
+  return hipGetLastError();
+}
+
 // Elements per thread: 8 (T = N / 8 <= 1024 threads) up to 8192 points, 16 at 16384.
 int reg_elems(int64_t n) { return n > 8192 ? 16 : 8; }
 

@@ -89,13 +89,21 @@ def test_reference_plans_bit_exact(lf, oracle, N, rs, split):
 
 @pytest.mark.parametrize("N,rs,split", [(4096, [16, 16, 16], 1), (4096, [16, 16, 16], 2),
                                         (2048, [2, 4, 8, 16, 2], 3), (30030, [13, 11, 7, 6, 5], 2),
-                                        (65536, [16, 16, 16, 16], 2), (96, [3, 2, 16], 1)])
+                                        (65536, [16, 16, 16, 16], 2), (96, [3, 2, 16], 1),
+                                        (5120, [8, 8, 2, 8, 5], 3), (10240, [5, 8, 8, 8, 4], 2),
+                                        (6144, [6, 2, 8, 8, 8], 2),
+                                        (32768, [8, 8, 2, 8, 8, 4], 3),
+                                        (131072, [8, 8, 4, 8, 8, 8], 3)])
 def test_stage_splits_bit_exact(lf, oracle, N, rs, split):
-    """The same schedule cut into kernel stages at different points gives the same bits."""
+    """The same schedule cut into kernel stages at different points gives the same bits,
+    including register-resident stages with a partial last tile (5120: 40 columns of 128 in
+    tiles of 16; 10240: 40 residue classes of 256 in tiles of 16; 6144: 12 residue classes of
+    512 in tiles of 8), both directions."""
     x = oracle.lcg_input(N, seed=77 + N, howmany=3)
-    plan = lf.Plan(N, lf.FORWARD, 3, radices=rs, split=split)
-    got = run(lf, plan, x)
-    assert bits_equal(got, oracle.ref_transform(True, rs, x))
+    for fwd in (True, False):
+        plan = lf.Plan(N, lf.FORWARD if fwd else lf.BACKWARD, 3, radices=rs, split=split)
+        got = run(lf, plan, x)
+        assert bits_equal(got, oracle.ref_transform(fwd, rs, x)), (N, rs, split, fwd)
 
 
 @pytest.mark.parametrize("N,rs", [(2048, [8, 8, 8, 4]), (4096, [8, 8, 8, 8]), (8192, [8, 8, 8, 8, 2]),
