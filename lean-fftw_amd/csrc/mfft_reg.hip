@@ -15,6 +15,7 @@
 // pass's columns. Up to 8192 points the exchange moves whole complex values (N padded
 // double2, 136 KiB at 8192); at 16384 it moves the real parts, then the imaginary parts,
 // through one buffer of N doubles (132 KiB), half of what the transform occupies.
+#include <cstdlib>
 #include <mutex>
 #include <set>
 
@@ -236,7 +237,7 @@ __global__ __launch_bounds__(reg_block<T>(), 4) void reg_kernel(const double2 *i
 }
 
 // ---------------------------------------------------------------- fixed schedules
-// The default schedules of 2048 ... 16384 points as compile-time radix sequences: ns, the
+// The default schedules of 64 ... 16384 points as compile-time radix sequences: ns, the
 // twiddle flag and the exchange offsets become constants and the passes straight-line code,
 // which takes the kernel from about 104 to about 61 registers at 8 elements per thread (and
 // lets 16 elements per thread run radix 8 and 16 without spilling). With the exchange of
@@ -255,18 +256,26 @@ __device__ __forceinline__ void fixed_passes(double2 (&v)[E], int t, void *lds,
   }
 }
 
+// Below 2048 points a workgroup runs G = 256 / T transforms, each with its own LDS region,
+// and exchanges whole complex values (LDS is not the limit there); from 2048 on one transform
+// and the real/imaginary exchange.
 template <bool FWD, int T, int E, bool SQ, int... Rs>
-__global__ __launch_bounds__(T, 4) void fixed_kernel(const double2 *in, double2 *out,
-                                                    const double2 *__restrict__ tw,
-                                                    const StageArgs a) {
+__global__ __launch_bounds__(reg_block<T>(), 4) void fixed_kernel(const double2 *in, double2 *out,
+                                                                 const double2 *__restrict__ tw,
+                                                                 const StageArgs a) {
+  constexpr int G = reg_block<T>() / T;
+  constexpr bool HALF = G == 1;
   extern __shared__ double2 lds_c[];
-  const int t = threadIdx.x;
-  const int64_t h = blockIdx.x;
+  const int t = G == 1 ? (int)threadIdx.x : (int)threadIdx.x % T;
+  const int g = G == 1 ? 0 : (int)threadIdx.x / T;
+  void *lds_d = lds_c + g * lds_len(T * E);
+  const int64_t h = (int64_t)blockIdx.x * G + g;
+  const bool valid = G == 1 || h < a.nsub;  // one transform per workgroup: the grid is the batch
   double2 v[E];
-  reg_load<T, E>(v, a, in + h * a.in_g, t, true);
-  fixed_passes<FWD, T, E, true, 0, 1, Rs...>(v, t, lds_c, a, tw);
-  const double sumsq = reg_store<T, E>(v, a, out + h * a.out_g, t, true);
-  if constexpr (SQ) block_partial<T>(sumsq, a.sumsq + blockIdx.x);
+  reg_load<T, E>(v, a, in + (valid ? h : 0) * a.in_g, t, valid);
+  fixed_passes<FWD, T, E, HALF, 0, 1, Rs...>(v, t, lds_d, a, tw);
+  const double sumsq = reg_store<T, E>(v, a, out + (valid ? h : 0) * a.out_g, t, valid);
+  if constexpr (SQ) block_partial<reg_block<T>()>(sumsq, a.sumsq + blockIdx.x);
 }
 
 // The fixed schedules: N, elements per thread, radix sequence, and the kernels by
@@ -283,6 +292,11 @@ struct FixedSchedule {
    {(const void *)fixed_kernel<true, TT, EE, false, __VA_ARGS__>,                             \
     (const void *)fixed_kernel<true, TT, EE, true, __VA_ARGS__>}}
 const FixedSchedule kFixed[] = {
+    {64, 8, 2, {8, 8}, MFFT_FX(8, 8, 8, 8)},
+    {128, 8, 3, {8, 8, 2}, MFFT_FX(16, 8, 8, 8, 2)},
+    {256, 8, 3, {8, 8, 4}, MFFT_FX(32, 8, 8, 8, 4)},
+    {512, 8, 3, {8, 8, 8}, MFFT_FX(64, 8, 8, 8, 8)},
+    {1024, 8, 4, {8, 8, 8, 2}, MFFT_FX(128, 8, 8, 8, 8, 2)},
     {2048, 8, 4, {8, 8, 8, 4}, MFFT_FX(256, 8, 8, 8, 8, 4)},
     {4096, 8, 4, {8, 8, 8, 8}, MFFT_FX(512, 8, 8, 8, 8, 8)},
     {8192, 8, 5, {8, 8, 8, 8, 2}, MFFT_FX(1024, 8, 8, 8, 8, 8, 2)},
@@ -293,6 +307,11 @@ const FixedSchedule kFixed[] = {
 #undef MFFT_FX
 
 const FixedSchedule *fixed_schedule(int64_t n, const int *radices, int nradix) {
+  static const int64_t min_n = [] {  // MFFT_FIXED_MIN: smallest size with a fixed schedule
+    const char *e = std::getenv("MFFT_FIXED_MIN");
+    return e ? (int64_t)std::atoll(e) : (int64_t)0;
+  }();
+  if (n < min_n) return nullptr;
   for (const auto &f : kFixed) {
     if (f.n != n || f.npass != nradix) continue;
     bool same = true;
@@ -530,7 +549,9 @@ hipError_t launch_reg(bool fwd, const StageArgs &a, const double2 *in, double2 *
   if (const FixedSchedule *f = fixed_schedule(a.L, rs, a.npass)) {
     const void *k = f->k[fwd ? 1 : 0][a.sumsq ? 1 : 0];
     const int T = a.L / f->e;
-    const size_t lds = (size_t)reg_lds_doubles(a.L) * sizeof(double);
+    const int BT = T < 256 ? 256 : T, G = BT / T;  // as reg_block<T>()
+    const size_t lds = G == 1 ? (size_t)reg_lds_doubles(a.L) * sizeof(double)
+                              : (size_t)G * lds_len(a.L) * sizeof(double2);
     {
       static std::mutex mu;
       static std::set<const void *> done;
@@ -539,7 +560,8 @@ hipError_t launch_reg(bool fwd, const StageArgs &a, const double2 *in, double2 *
         (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytesMax);
     }
     void *args[] = {(void *)&in, (void *)&out, (void *)&tw, (void *)&a};
-    (void)hipLaunchKernel(k, dim3((unsigned)a.nsub), dim3(T), args, lds, stream);
+    const int64_t nblk = (a.nsub + G - 1) / G;
+    (void)hipLaunchKernel(k, dim3((unsigned)nblk), dim3(BT), args, lds, stream);
     return hipGetLastError();
   }
   const int E = reg_elems(a.L), T = a.L / E;
