@@ -61,11 +61,36 @@ mfft_plan *mfft_plan_many(int64_t n, int sign, int64_t howmany, int64_t istride,
  * Replaces lfa_plan_measure(N, fwd, in, out) (with howmany = 1). */
 mfft_plan *mfft_plan_dft_1d(int64_t n, int sign, int64_t howmany, unsigned flags);
 
+/* Real-input forward transform: howmany contiguous real arrays of n doubles -> howmany
+ * contiguous arrays of n/2 + 1 interleaved complex values, F_k = sum_j x_j w^(jk), k <= n/2.
+ * Computed as the reference's generic engine does (LineOp.r2c, Problem.lean:128-129,
+ * :144-146): the complex forward DFT of (x_j, 0), first n/2 + 1 outputs. in != out.
+ * Replaces the rdft2 forward rank-1 case of planProblemCore (Problem.lean:227-234). */
+mfft_plan *mfft_plan_r2c(int64_t n, int64_t howmany, unsigned flags);
+
+/* Real-output backward transform (FFTW's c2r, unnormalised): howmany arrays of n/2 + 1
+ * complex values -> howmany real arrays of n doubles, the real part of the backward DFT of
+ * the Hermitian extension (hermExt, Problem.lean:138-140; LineOp.c2r :147-149). in != out.
+ * Replaces the rdft2 backward rank-1 case of planProblemCore (Problem.lean:235-238). */
+mfft_plan *mfft_plan_c2r(int64_t n, int64_t howmany, unsigned flags);
+
+/* 2-D complex DFT of howmany contiguous row-major n0 x n1 arrays: the line DFT along
+ * axis 0 (length n0, stride n1), then along axis 1 (length n1, contiguous), the order of the
+ * reference's per-axis ops (planProblemCore .dft, Problem.lean:222-224; alongAxis
+ * :165-182). in == out allowed. Replaces the dft rank-2 case of lfftw_generic_plan. */
+mfft_plan *mfft_plan_dft_2d(int64_t n0, int64_t n1, int sign, int64_t howmany, unsigned flags);
+
+/* 0 = complex 1-D, 1 = r2c, 2 = c2r, 3 = 2-D complex; -1 for NULL. The composite kinds
+ * (1..3) reject the weighted and sum-of-squares entry points with MFFT_ERR_UNSUPPORTED. */
+int mfft_plan_kind(const mfft_plan *p);
+
 /* Plan from the reference's generic problem spec (FFI.lean:13-21):
  * [kind, sign, rank, vrank, inRe, inIm, outRe, outIm, (n,is,os) x rank, (n,is,os) x vrank]
- * in units of doubles. Supported: kind 0 (complex DFT), rank 1, vrank 0 or 1, interleaved
- * (inIm = inRe + 1, outIm = outRe + 1, even strides). Offsets are applied by the caller
- * through the pointers given to mfft_execute_spec. Returns NULL otherwise.
+ * in units of doubles. Supported, vrank 0 or 1: kind 0 (complex DFT) rank 1, interleaved
+ * (inIm = inRe + 1, outIm = outRe + 1, even strides); kind 0 rank 2 and kind 1 (rdft2)
+ * rank 1 in contiguous interleaved layouts (mfft_plan_dft_2d, mfft_plan_r2c,
+ * mfft_plan_c2r). Offsets are applied by the caller through the pointers given to
+ * mfft_execute_spec. Returns NULL otherwise.
  * Replaces lfftw_generic_plan (FFI.lean:50-52). */
 mfft_plan *mfft_plan_spec(const double *spec, size_t len, unsigned flags);
 

@@ -55,6 +55,10 @@ def lib():
                 "mfft_plan_many": (vp, [i64, ctypes.c_int, i64, i64, i64, i64, i64, ctypes.c_uint]),
                 "mfft_plan_dft_1d": (vp, [i64, ctypes.c_int, i64, ctypes.c_uint]),
                 "mfft_plan_spec": (vp, [dp, ctypes.c_size_t, ctypes.c_uint]),
+                "mfft_plan_r2c": (vp, [i64, i64, ctypes.c_uint]),
+                "mfft_plan_c2r": (vp, [i64, i64, ctypes.c_uint]),
+                "mfft_plan_dft_2d": (vp, [i64, i64, ctypes.c_int, i64, ctypes.c_uint]),
+                "mfft_plan_kind": (ctypes.c_int, [vp]),
                 "mfft_plan_with_radices": (vp, [i64, ctypes.c_int, i64, ctypes.POINTER(ctypes.c_int),
                                                 ctypes.c_int, ctypes.c_int]),
                 "mfft_execute": (ctypes.c_int, [vp, vp, vp, vp]),
@@ -300,6 +304,100 @@ def plan_problem(spec) -> SpecPlan | None:
         return SpecPlan(spec)
     except MfftError:
         return None
+
+
+class _CompositePlan:
+    """Shared handle logic of the composite plans (r2c, c2r, 2-D)."""
+
+    _h = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.mfft_destroy_plan(h)
+            self._h = None
+
+    @property
+    def radices(self) -> list[int]:
+        buf = (ctypes.c_int * 32)()
+        n = lib().mfft_plan_radices(self._h, buf, 32)
+        return list(buf[:n])
+
+    @staticmethod
+    def _check(t, dtype, numel, what):
+        if t.dtype != dtype or not t.is_cuda or not t.is_contiguous():
+            raise TypeError(f"{what} must be a contiguous {dtype} CUDA tensor")
+        if t.numel() < numel:
+            raise ValueError(f"{what} holds {t.numel()} elements, the plan needs {numel}")
+
+    def _exec(self, inp, out, stream):
+        _check(lib().mfft_execute(self._h, ctypes.c_void_p(inp.data_ptr()),
+                                  ctypes.c_void_p(out.data_ptr()), _stream_ptr(stream)),
+               "mfft_execute")
+        return out
+
+
+class RealPlan(_CompositePlan):
+    """Batched real-data transforms of n points (the reference's rdft2, rank 1):
+    forward (r2c) float64 [howmany, n] -> complex128 [howmany, n//2 + 1]; backward (c2r,
+    unnormalised) complex128 [howmany, n//2 + 1] -> float64 [howmany, n]. Input and output
+    are distinct tensors."""
+
+    def __init__(self, n, sign=FORWARD, howmany=1, flags=ESTIMATE):
+        L = lib()
+        self.n, self.sign, self.howmany = int(n), int(sign), int(howmany)
+        self.nh = self.n // 2 + 1
+        f = L.mfft_plan_r2c if self.sign == FORWARD else L.mfft_plan_c2r
+        h = f(self.n, self.howmany, flags)
+        if not h:
+            raise MfftError(f"unsupported real problem: n={n} sign={sign} howmany={howmany}")
+        self._h = ctypes.c_void_p(h)
+
+    def call(self, inp, out, stream=None):
+        import torch
+
+        if inp.data_ptr() == out.data_ptr():
+            raise ValueError("real transforms are out of place")
+        fwd = self.sign == FORWARD
+        self._check(inp, torch.float64 if fwd else torch.complex128,
+                    self.howmany * (self.n if fwd else self.nh), "input")
+        self._check(out, torch.complex128 if fwd else torch.float64,
+                    self.howmany * (self.nh if fwd else self.n), "output")
+        return self._exec(inp, out, stream)
+
+    def exec(self, inp, stream=None):
+        import torch
+
+        if self.sign == FORWARD:
+            out = torch.empty((self.howmany, self.nh), dtype=torch.complex128, device=inp.device)
+        else:
+            out = torch.empty((self.howmany, self.n), dtype=torch.float64, device=inp.device)
+        return self.call(inp, out, stream)
+
+
+class Plan2D(_CompositePlan):
+    """2-D complex transforms of howmany contiguous row-major n0 x n1 complex128 arrays:
+    the DFT along axis 0, then along axis 1 (inp may be out)."""
+
+    def __init__(self, n0, n1, sign=FORWARD, howmany=1, flags=ESTIMATE):
+        self.n0, self.n1, self.sign, self.howmany = int(n0), int(n1), int(sign), int(howmany)
+        h = lib().mfft_plan_dft_2d(self.n0, self.n1, self.sign, self.howmany, flags)
+        if not h:
+            raise MfftError(f"unsupported 2-D problem: {n0} x {n1} sign={sign} howmany={howmany}")
+        self._h = ctypes.c_void_p(h)
+
+    def call(self, inp, out, stream=None):
+        import torch
+
+        size = self.howmany * self.n0 * self.n1
+        self._check(inp, torch.complex128, size, "input")
+        self._check(out, torch.complex128, size, "output")
+        return self._exec(inp, out, stream)
+
+    def exec(self, inp, stream=None):
+        import torch
+
+        return self.call(inp, torch.empty_like(inp), stream)
 
 
 class RaggedPlan:

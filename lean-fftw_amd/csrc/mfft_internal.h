@@ -116,6 +116,18 @@ int reg_elems(int64_t n);  // elements per thread of the register-resident kerne
 hipError_t launch_reg(bool fwd, const StageArgs &a, const double2 *in, double2 *out,
                       const double2 *tw, hipStream_t stream);
 
+// Element maps of the composite transforms (mfft_real.hip): embedding of a real batch
+// (howmany x n doubles) as complex, the first n/2 + 1 of each transform, the Hermitian
+// extension of half-complex transforms, the real parts.
+hipError_t launch_embed_real(const double *x, double2 *work, int64_t n, int64_t howmany,
+                             hipStream_t st);
+hipError_t launch_extract_half(const double2 *work, double2 *out, int64_t n, int64_t howmany,
+                               hipStream_t st);
+hipError_t launch_herm_ext(const double2 *in, double2 *work, int64_t n, int64_t howmany,
+                           hipStream_t st);
+hipError_t launch_real_part(const double2 *work, double *out, int64_t n, int64_t howmany,
+                            hipStream_t st);
+
 // Launch one stage over `ntrans` transforms; returns a hipError_t.
 hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,
                         double2 *out, const double2 *tw, hipStream_t stream);

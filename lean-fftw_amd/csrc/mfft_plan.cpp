@@ -49,6 +49,15 @@ struct mfft_plan {
   int device = 0;
   int64_t spec_in_off = 0, spec_out_off = 0;  // doubles, for mfft_plan_spec plans
   bool allow_big = true;  // may use the large-tile single-stage variant
+  // composite plans, built from complex plans (parts) the way the reference's generic engine
+  // builds them (LeanFFTW/Core/Problem.lean:215-236): 0 complex DFT; 1 real to half complex
+  // (r2c, part 0 the forward DFT on the embedded batch in d_work); 2 half complex to real (c2r,
+  // part 0 the backward DFT on the Hermitian extension in d_work); 3 two-dimensional complex
+  // DFT of n0 x n1 row-major arrays (part 0 along axis 0, then part 1 along axis 1)
+  int kind = 0;
+  mfft_plan *part[2] = {nullptr, nullptr};
+  double2 *d_work = nullptr;
+  int64_t n0 = 0, n1 = 0;
 };
 
 namespace {
@@ -206,11 +215,14 @@ int64_t reg_min() {
 
 bool reg_size(int64_t n) { return (n & (n - 1)) == 0 && n >= reg_min() && n <= 16384; }
 
-// Whether a size runs as a single kernel stage (else it is split into two).
-bool single_stage(int64_t n, bool allow_big) {
+// Whether a size runs as a single kernel stage (else it is split into two). The register-
+// resident kernel walks the element index with consecutive lanes, so it takes unit-stride
+// batches only (`contig`); strided batches run the stage kernel, which walks the batch.
+bool single_stage(int64_t n, bool allow_big, bool contig = true) {
   const bool pow2 = (n & (n - 1)) == 0;
   return n <= mfft::LDS_ELEMS ||
-         (allow_big && (n <= (pow2 ? mfft::BIG_ELEMS : mfft::BIG_ELEMS / 2) || reg_size(n)));
+         (allow_big &&
+          (n <= (pow2 ? mfft::BIG_ELEMS : mfft::BIG_ELEMS / 2) || (contig && reg_size(n))));
 }
 
 hipError_t build_twiddles(mfft_plan *p) {
@@ -340,7 +352,8 @@ bool build_stages(mfft_plan *p, int split) {
   std::vector<int64_t> ns(s + 1, 1);
   for (int t = 0; t < s; ++t) ns[t + 1] = ns[t] * rs[t];
   if (split <= 0 || split >= s) {
-    const bool reg = p->allow_big && reg_size(N) && mfft::reg_supported(N, rs.data(), s);
+    const bool reg = p->allow_big && reg_size(N) && p->istride == 1 && p->ostride == 1 &&
+                     mfft::reg_supported(N, rs.data(), s);
     const bool big = !reg && N > mfft::LDS_ELEMS && p->allow_big &&
                      N <= ((N & (N - 1)) == 0 ? mfft::BIG_ELEMS : mfft::BIG_ELEMS / 2);
     if ((N > mfft::LDS_ELEMS && !big && !reg) || s > mfft::MAX_PASSES) return false;
@@ -495,6 +508,79 @@ int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
   return MFFT_SUCCESS;
 }
 
+// Composite plans: the element maps of mfft_real.hip around the complex parts.
+int run_composite(const mfft_plan *p, const void *in, void *out, hipStream_t st) {
+  if (!in || !out) return MFFT_ERR_INVALID;
+  const int64_t hm = p->howmany;
+  if (hm == 0) return MFFT_SUCCESS;
+  switch (p->kind) {
+    case 1: {  // r2c: embed, forward DFT in place, first n/2 + 1 outputs
+      if (mfft::launch_embed_real((const double *)in, p->d_work, p->n, hm, st) != hipSuccess)
+        return MFFT_ERR_HIP;
+      const int rc = run(p->part[0], p->d_work, p->d_work, nullptr, nullptr, nullptr, st);
+      if (rc != MFFT_SUCCESS) return rc;
+      return mfft::launch_extract_half(p->d_work, (double2 *)out, p->n, hm, st) == hipSuccess
+                 ? MFFT_SUCCESS
+                 : MFFT_ERR_HIP;
+    }
+    case 2: {  // c2r: Hermitian extension, backward DFT in place, real parts
+      if (mfft::launch_herm_ext((const double2 *)in, p->d_work, p->n, hm, st) != hipSuccess)
+        return MFFT_ERR_HIP;
+      const int rc = run(p->part[0], p->d_work, p->d_work, nullptr, nullptr, nullptr, st);
+      if (rc != MFFT_SUCCESS) return rc;
+      return mfft::launch_real_part(p->d_work, (double *)out, p->n, hm, st) == hipSuccess
+                 ? MFFT_SUCCESS
+                 : MFFT_ERR_HIP;
+    }
+    case 3: {  // 2-D: axis 0 (columns, stride n1) of every array, then axis 1 (rows)
+      const int64_t sz = p->n0 * p->n1;
+      for (int64_t h = 0; h < hm; ++h) {
+        const int rc = run(p->part[0], (const double2 *)in + h * sz, (double2 *)out + h * sz,
+                           nullptr, nullptr, nullptr, st);
+        if (rc != MFFT_SUCCESS) return rc;
+      }
+      return run(p->part[1], (const double2 *)out, (double2 *)out, nullptr, nullptr, nullptr, st);
+    }
+    default:
+      return MFFT_ERR_INVALID;
+  }
+}
+
+// Bytes of the input and output arrays of a composite plan (host-pointer execution).
+void composite_bytes(const mfft_plan *p, size_t &in_b, size_t &out_b) {
+  const size_t hm = (size_t)p->howmany, n = (size_t)p->n, nh = (size_t)(p->n / 2 + 1);
+  if (p->kind == 1) {
+    in_b = hm * n * sizeof(double);
+    out_b = hm * nh * sizeof(double2);
+  } else if (p->kind == 2) {
+    in_b = hm * nh * sizeof(double2);
+    out_b = hm * n * sizeof(double);
+  } else {
+    in_b = out_b = hm * n * sizeof(double2);
+  }
+}
+
+mfft_plan *composite_1d(int kind, int64_t n, int64_t howmany, unsigned flags) {
+  if (howmany < 0) return nullptr;
+  mfft_plan *inner = mfft_plan_dft_1d(n, kind == 1 ? MFFT_FORWARD : MFFT_BACKWARD,
+                                      std::max<int64_t>(howmany, 1), flags);
+  if (!inner) return nullptr;
+  auto *p = new mfft_plan;
+  p->kind = kind;
+  p->n = n;
+  p->sign = kind == 1 ? -1 : 1;
+  p->howmany = howmany;
+  p->part[0] = inner;
+  p->radices = inner->radices;
+  p->stage_passes = inner->stage_passes;
+  if (howmany > 0 &&
+      hipMalloc(&p->d_work, (size_t)howmany * n * sizeof(double2)) != hipSuccess) {
+    mfft_destroy_plan(p);
+    return nullptr;
+  }
+  return p;
+}
+
 // ------------------------------------------------------------------ autotuner (MFFT_MEASURE)
 //
 // The model of the reference's measuring planner (LeanFFTW/Asm/Planner.lean:224-286:
@@ -529,10 +615,10 @@ std::vector<std::vector<int>> seq_candidates(int64_t L) {
   return out;
 }
 
-std::vector<Cand> schedule_candidates(int64_t n, bool allow_big) {
+std::vector<Cand> schedule_candidates(int64_t n, bool allow_big, bool contig) {
   std::vector<Cand> out;
   const bool pow2 = (n & (n - 1)) == 0;
-  if (single_stage(n, allow_big))
+  if (single_stage(n, allow_big, contig))
     for (auto &v : seq_candidates(n)) out.push_back({v, 0});
   if (n > mfft::LDS_ELEMS) {
     std::vector<std::pair<double, int64_t>> splits;
@@ -667,7 +753,8 @@ bool measure(const mfft_plan *shape, Cand &best) {
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
   double best_us = 1e300;
-  for (const Cand &c : schedule_candidates(n, shape->allow_big)) {
+  for (const Cand &c : schedule_candidates(n, shape->allow_big,
+                                           shape->istride == 1 && shape->ostride == 1)) {
     if (!shape->allow_big && !all_mid(c.rs)) continue;  // ragged classes: the mid kernel
     mfft_plan *q = clone_shape(shape, hm);
     q->radices = c.rs;
@@ -730,8 +817,9 @@ mfft_plan *plan_many_internal(int64_t n, int sign, int64_t howmany, int64_t istr
       }
     }
   }
-  if (single_stage(n, allow_big)) {
-    p->radices = allow_big && reg_size(n) ? reg_radices(n) : choose_radices(n, !allow_big);
+  const bool contig = istride == 1 && ostride == 1;
+  if (single_stage(n, allow_big, contig)) {
+    p->radices = allow_big && contig && reg_size(n) ? reg_radices(n) : choose_radices(n, !allow_big);
   } else {
     int64_t P;
     if (!choose_split(n, P)) {
@@ -830,7 +918,37 @@ mfft_plan *mfft_plan_spec(const double *spec, size_t len, unsigned flags) {
   if (!spec || len < 8) return nullptr;
   auto gi = [&](size_t i) { return (int64_t)spec[i]; };
   const int64_t kind = gi(0), sign = gi(1), rank = gi(2), vrank = gi(3);
-  if (kind != 0 || rank != 1 || vrank < 0 || vrank > 1) return nullptr;
+  if (vrank < 0 || vrank > 1 || rank < 1 || rank > 2) return nullptr;
+  if (len < (size_t)(8 + 3 * (rank + vrank))) return nullptr;
+  if (kind == 1 || rank == 2) {
+    // contiguous layouts only (else NULL: the caller keeps its own path)
+    const int64_t inRe = gi(4), inIm = gi(5), outRe = gi(6), outIm = gi(7);
+    const int64_t hm = vrank ? gi(8 + 3 * rank) : 1;
+    mfft_plan *p = nullptr;
+    if (kind == 1 && rank == 1) {
+      const int64_t n = gi(8), is = gi(9), os = gi(10), nh = n / 2 + 1;
+      const bool fwd = sign < 0;
+      const int64_t vis = vrank ? gi(12) : (fwd ? n : 2 * nh);
+      const int64_t vos = vrank ? gi(13) : (fwd ? 2 * nh : n);
+      const bool ok = fwd ? (is == 1 && os == 2 && vis == n && vos == 2 * nh && outIm == outRe + 1)
+                          : (is == 2 && os == 1 && vis == 2 * nh && vos == n && inIm == inRe + 1);
+      if (!ok) return nullptr;
+      p = fwd ? mfft_plan_r2c(n, hm, flags) : mfft_plan_c2r(n, hm, flags);
+    } else if (kind == 0 && rank == 2) {
+      const int64_t n0 = gi(8), n1 = gi(11);
+      const bool ok = inIm == inRe + 1 && outIm == outRe + 1 && gi(9) == 2 * n1 &&
+                      gi(10) == 2 * n1 && gi(12) == 2 && gi(13) == 2 &&
+                      (!vrank || (gi(15) == 2 * n0 * n1 && gi(16) == 2 * n0 * n1));
+      if (!ok) return nullptr;
+      p = mfft_plan_dft_2d(n0, n1, sign < 0 ? -1 : 1, hm, flags);
+    }
+    if (p) {
+      p->spec_in_off = inRe;
+      p->spec_out_off = outRe;
+    }
+    return p;
+  }
+  if (kind != 0) return nullptr;
   if (len < (size_t)(8 + 3 * (rank + vrank))) return nullptr;
   const int64_t inRe = gi(4), inIm = gi(5), outRe = gi(6), outIm = gi(7);
   if (inIm != inRe + 1 || outIm != outRe + 1) return nullptr;  // interleaved only
@@ -852,33 +970,94 @@ mfft_plan *mfft_plan_spec(const double *spec, size_t len, unsigned flags) {
 
 int mfft_execute(const mfft_plan *p, const void *in, void *out, void *stream) {
   if (!p) return MFFT_ERR_INVALID;
+  if (p->kind != 0) {
+    if (p->kind != 3 && in == out) return MFFT_ERR_UNSUPPORTED;  // different element types
+    return run_composite(p, in, out, (hipStream_t)stream);
+  }
   return run(p, (const double2 *)in, (double2 *)out, nullptr, nullptr, nullptr, (hipStream_t)stream);
 }
+
+mfft_plan *mfft_plan_r2c(int64_t n, int64_t howmany, unsigned flags) {
+  return composite_1d(1, n, howmany, flags);
+}
+
+mfft_plan *mfft_plan_c2r(int64_t n, int64_t howmany, unsigned flags) {
+  return composite_1d(2, n, howmany, flags);
+}
+
+mfft_plan *mfft_plan_dft_2d(int64_t n0, int64_t n1, int sign, int64_t howmany, unsigned flags) {
+  if (n0 < 2 || n1 < 2 || howmany < 0 || (sign != -1 && sign != 1)) return nullptr;
+  mfft_plan *col = mfft_plan_many(n0, sign, n1, n1, 1, n1, 1, flags);
+  mfft_plan *row = col ? mfft_plan_many(n1, sign, n0 * std::max<int64_t>(howmany, 1), 1, n1, 1,
+                                        n1, flags)
+                       : nullptr;
+  if (!col || !row) {
+    mfft_destroy_plan(col);
+    mfft_destroy_plan(row);
+    return nullptr;
+  }
+  auto *p = new mfft_plan;
+  p->kind = 3;
+  p->n = n0 * n1;
+  p->n0 = n0;
+  p->n1 = n1;
+  p->sign = sign;
+  p->howmany = howmany;
+  p->part[0] = col;
+  p->part[1] = row;
+  p->radices = row->radices;
+  p->stage_passes = row->stage_passes;
+  return p;
+}
+
+int mfft_plan_kind(const mfft_plan *p) { return p ? p->kind : -1; }
 
 int mfft_execute_weighted(const mfft_plan *p, const void *in, void *out, const double *pre,
                           const double *post, void *stream) {
   if (!p) return MFFT_ERR_INVALID;
+  if (p->kind != 0) return MFFT_ERR_UNSUPPORTED;
   if ((pre || post) && (p->istride != 1 || p->ostride != 1)) return MFFT_ERR_UNSUPPORTED;
   return run(p, (const double2 *)in, (double2 *)out, pre, post, nullptr, (hipStream_t)stream);
 }
 
-int64_t mfft_plan_partials(const mfft_plan *p) { return p ? count_partials(p) : -1; }
+int64_t mfft_plan_partials(const mfft_plan *p) {
+  return p && p->kind == 0 ? count_partials(p) : -1;
+}
 
 int mfft_execute_weighted_sumsq(const mfft_plan *p, const void *in, void *out, const double *pre,
                                 const double *post, double *partials, void *stream) {
   if (!p || !partials) return MFFT_ERR_INVALID;
+  if (p->kind != 0) return MFFT_ERR_UNSUPPORTED;
   if ((pre || post) && (p->istride != 1 || p->ostride != 1)) return MFFT_ERR_UNSUPPORTED;
   return run(p, (const double2 *)in, (double2 *)out, pre, post, partials, (hipStream_t)stream);
 }
 
 int mfft_execute_spec(const mfft_plan *p, const double *in, double *out, void *stream) {
   if (!p || !in || !out) return MFFT_ERR_INVALID;
-  return run(p, (const double2 *)(in + p->spec_in_off), (double2 *)(out + p->spec_out_off),
-             nullptr, nullptr, nullptr, (hipStream_t)stream);
+  return mfft_execute(p, in + p->spec_in_off, out + p->spec_out_off, stream);
 }
 
 int mfft_execute_host(const mfft_plan *p, const double *in, double *out) {
   if (!p || !in || !out) return MFFT_ERR_INVALID;
+  if (p->kind != 0) {  // contiguous composite layouts
+    size_t ib = 0, ob = 0;
+    composite_bytes(p, ib, ob);
+    void *di = nullptr, *dout = nullptr;
+    if (hipMalloc(&di, ib ? ib : 16) != hipSuccess) return MFFT_ERR_ALLOC;
+    if (hipMalloc(&dout, ob ? ob : 16) != hipSuccess) {
+      (void)hipFree(di);
+      return MFFT_ERR_ALLOC;
+    }
+    int rc = hipMemcpy(di, in, ib, hipMemcpyHostToDevice) == hipSuccess ? MFFT_SUCCESS
+                                                                          : MFFT_ERR_HIP;
+    if (rc == MFFT_SUCCESS) rc = run_composite(p, di, dout, nullptr);
+    if (rc == MFFT_SUCCESS && (hipDeviceSynchronize() != hipSuccess ||
+                               hipMemcpy(out, dout, ob, hipMemcpyDeviceToHost) != hipSuccess))
+      rc = MFFT_ERR_HIP;
+    (void)hipFree(di);
+    (void)hipFree(dout);
+    return rc;
+  }
   // span of elements touched on each side
   auto span = [&](int64_t stride, int64_t dist) {
     const int64_t a = std::llabs(stride) * (p->n - 1) + std::llabs(dist) * (p->howmany - 1) + 1;
@@ -911,6 +1090,9 @@ void mfft_destroy_plan(mfft_plan *p) {
   if (!p) return;
   if (p->d_tw) (void)hipFree(p->d_tw);
   if (p->d_scratch) (void)hipFree(p->d_scratch);
+  if (p->d_work) (void)hipFree(p->d_work);
+  mfft_destroy_plan(p->part[0]);
+  mfft_destroy_plan(p->part[1]);
   delete p;
 }
 

@@ -331,3 +331,29 @@ def test_lcg_matches_reference_recurrence(oracle):
     for i in range(2 * 5000 * 3):
         s = (s * 6364136223846793005 + 1442695040888963407) & M
     assert big.view(np.float64).reshape(-1)[-1] == (s >> 11) / 9007199254740992.0 - 0.5
+
+
+def test_composite_maps_vs_numpy(oracle):
+    """The compositions the real-data and 2-D transforms are checked against (embedding,
+    first n/2 + 1 outputs, Hermitian extension, real part, axis 0 then axis 1; the generic
+    engine's LineOp.apply and alongAxis, Problem.lean:142-182) evaluated with the oracle
+    agree with numpy's independent rfft / irfft / fft2 (pocketfft)."""
+    rng = np.random.default_rng(4)
+    for n, rs in ((8, [8]), (15, [15]), (1000, [8, 5, 5, 5]), (1024, [8, 4, 4, 8])):
+        x = rng.standard_normal((3, n))
+        half = oracle.ref_transform(True, rs, x.astype(np.complex128))[:, : n // 2 + 1]
+        ref = np.fft.rfft(x, axis=1)
+        assert np.linalg.norm(half - ref) / np.linalg.norm(ref) < 1e-14
+        h = ref + 1j * rng.standard_normal(ref.shape) * (np.arange(ref.shape[1]) == 0)
+        k = np.arange(n)
+        ext = np.where(k <= n // 2, h[:, np.minimum(k, n // 2)],
+                       np.conj(h[:, np.minimum((n - k) % n, n // 2)]))
+        back = oracle.ref_transform(False, rs, ext).real
+        assert np.linalg.norm(back - n * x) / np.linalg.norm(n * x) < 1e-14
+    n0, n1 = 12, 16
+    x = rng.standard_normal((2, n0, n1)) + 1j * rng.standard_normal((2, n0, n1))
+    cols = np.ascontiguousarray(x.transpose(0, 2, 1)).reshape(-1, n0)
+    a = oracle.ref_transform(True, [12], cols).reshape(2, n1, n0).transpose(0, 2, 1)
+    b = oracle.ref_transform(True, [16], np.ascontiguousarray(a).reshape(-1, n1)).reshape(x.shape)
+    ref = np.fft.fft2(x)
+    assert np.linalg.norm(b - ref) / np.linalg.norm(ref) < 1e-14
