@@ -392,6 +392,59 @@ def bench_autograd(args, torch, lf):
                        "fwd_fft_GB/s": round(b / tg_f / 1e9, 1)}}
 
 
+def bench_real(args, torch, lf):
+    """Real-data forward transforms (r2c) of 2^4 .. 2^18 points, batch 2^24/N: the
+    reference's rdft2 problems, computed through the complex path. Bytes are the ideal
+    8N in + 16(N/2+1) out per transform."""
+    sizes = args.sizes or [2 ** k for k in range(4, 19)]
+    per, tot_b, tot_t = [], 0.0, 0.0
+    for N in sizes:
+        hm = (1 << 24) // N
+        g = torch.Generator("cuda:0").manual_seed(N)
+        x = torch.rand((hm, N), dtype=torch.float64, device="cuda:0", generator=g) - 0.5
+        y = torch.empty((hm, N // 2 + 1), dtype=torch.complex128, device="cuda:0")
+        p = lf.RealPlan(N, lf.FORWARD, hm)
+        t = time_steps(torch, lambda: p.call(x, y), args.steps, args.warmup)
+        b = (8.0 * N + 16.0 * (N // 2 + 1)) * hm
+        per.append({"N": N, "batch": hm, "radices": p.radices, "us": round(t * 1e6, 2),
+                    "GB/s": round(b / t / 1e9, 1)})
+        tot_b += b
+        tot_t += t
+        del x, y
+    v = tot_b / tot_t / 1e9
+    return {"metric": "real-data forward transforms (r2c) GB/s", "value": round(v, 2),
+            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(tot_t * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "r2c FP64, N = 2^4..2^18, batch 2^24/N, out of place"},
+            "per_size": per}
+
+
+def bench_2d(args, torch, lf):
+    """2-D complex forward transforms ocfNxN (N = 64 .. 4096), 2^24 elements per step."""
+    sizes = args.sizes or [64, 128, 256, 512, 1024, 2048, 4096]
+    per, tot_b, tot_t = [], 0.0, 0.0
+    for N in sizes:
+        hm = max(1, (1 << 24) // (N * N))
+        x = uniform_complex(torch, (hm, N, N), N)
+        y = torch.empty_like(x)
+        p = lf.Plan2D(N, N, lf.FORWARD, hm)
+        t = time_steps(torch, lambda: p.call(x, y), args.steps, args.warmup)
+        b = 32.0 * N * N * hm
+        per.append({"N": N, "batch": hm, "us": round(t * 1e6, 2), "GB/s": round(b / t / 1e9, 1)})
+        tot_b += b
+        tot_t += t
+        del x, y
+    v = tot_b / tot_t / 1e9
+    return {"metric": "2-D complex transforms GB/s", "value": round(v, 2), "unit": "GB/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(tot_t * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "ocfNxN: 2-D complex FP64 forward, N = 64..4096, "
+                                   "2^24 elements per size, out of place"},
+            "per_size": per}
+
+
 def bench_copy(args, torch, lf):
     """A plain device copy of 2^24 complex128 (256 MB read + 256 MB written): the practical
     HBM ceiling, and the reference launch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE."""
@@ -414,7 +467,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="sweep",
-                    choices=["sweep", "single", "mixed", "ragged", "autograd", "copy"])
+                    choices=["sweep", "single", "mixed", "ragged", "autograd", "copy", "real",
+                             "2d"])
     ap.add_argument("--sizes", type=int, nargs="*", help="restrict the sweep to these sizes")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -429,7 +483,8 @@ def main():
 
     lf = load_package()
     fn = {"sweep": bench_sweep, "single": bench_single, "mixed": bench_mixed,
-          "ragged": bench_ragged, "autograd": bench_autograd, "copy": bench_copy}[args.config]
+          "ragged": bench_ragged, "autograd": bench_autograd, "copy": bench_copy,
+          "real": bench_real, "2d": bench_2d}[args.config]
     res = fn(args, torch, lf)
     print(json.dumps(res), flush=True)
 

@@ -58,6 +58,9 @@ struct mfft_plan {
   mfft_plan *part[2] = {nullptr, nullptr};
   double2 *d_work = nullptr;
   int64_t n0 = 0, n1 = 0;
+  // an outer batch of single-stage plans: `outer` repetitions of the whole batch, the k-th
+  // at in + k * outer_in, out + k * outer_out (the arrays of a 2-D plan's axis-0 part)
+  int64_t outer = 1, outer_in = 0, outer_out = 0;
 };
 
 namespace {
@@ -360,8 +363,21 @@ bool build_stages(mfft_plan *p, int split) {
     StageArgs a{};
     a.L = (int)N;
     a.npass = s;
-    a.G = big ? 1 : (int)std::max<int64_t>(1, std::min<int64_t>(mfft::LDS_ELEMS / N, p->howmany));
-    set_tile(a, big, rs);
+    // a batch whose transforms are adjacent in memory (the columns of a row-major array) is
+    // tiled like a two-stage plan's column stage: up to 8 transforms (128-byte rows) per
+    // workgroup, in the register-resident stage kernel where a schedule exists
+    const bool batch_fast = std::llabs(p->idist) < std::llabs(p->istride) ||
+                            std::llabs(p->odist) < std::llabs(p->ostride);
+    if (batch_fast && !reg) {
+      strided_tile(a, N, std::max<int64_t>(1, p->howmany), p->allow_big, rs);
+      if (p->allow_big && std::llabs(p->idist) < std::llabs(p->istride) &&
+          std::llabs(p->odist) < std::llabs(p->ostride))
+        reg_stage(a, rs, std::max<int64_t>(1, p->howmany));
+    } else {
+      a.G = big ? 1
+                : (int)std::max<int64_t>(1, std::min<int64_t>(mfft::LDS_ELEMS / N, p->howmany));
+      set_tile(a, big, rs);
+    }
     for (int t = 0; t < s; ++t) fill_pass(a.pass[t], rs[t], ns[t], ns[t], 1, a.L, a.G);
     a.nsub = p->howmany;
     a.in_g = p->idist;
@@ -488,8 +504,18 @@ int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
     a.pre = pre;
     a.post = post;
     a.sumsq = sumsq;
-    return mfft::launch_stage(fwd, a, 1, in, out, p->d_tw, st) == hipSuccess ? MFFT_SUCCESS
-                                                                            : MFFT_ERR_HIP;
+    if (p->outer == 1)
+      return mfft::launch_stage(fwd, a, 1, in, out, p->d_tw, st) == hipSuccess ? MFFT_SUCCESS
+                                                                              : MFFT_ERR_HIP;
+    a.in_h = p->outer_in;  // grid.y walks the outer batch
+    a.out_h = p->outer_out;
+    for (int64_t o0 = 0; o0 < p->outer; o0 += 65535) {
+      const int64_t oc = std::min<int64_t>(65535, p->outer - o0);
+      if (mfft::launch_stage(fwd, a, oc, in + o0 * p->outer_in, out + o0 * p->outer_out,
+                             p->d_tw, st) != hipSuccess)
+        return MFFT_ERR_HIP;
+    }
+    return MFFT_SUCCESS;
   }
   const int64_t nxb = (p->stages[1].nsub + p->stages[1].G - 1) / p->stages[1].G;
   for (int64_t h0 = 0; h0 < p->howmany; h0 += p->chunk_h) {
@@ -534,7 +560,9 @@ int run_composite(const mfft_plan *p, const void *in, void *out, hipStream_t st)
     }
     case 3: {  // 2-D: axis 0 (columns, stride n1) of every array, then axis 1 (rows)
       const int64_t sz = p->n0 * p->n1;
-      for (int64_t h = 0; h < hm; ++h) {
+      // one call over every array when the axis-0 part carries the outer batch
+      const int64_t calls = p->part[0]->outer > 1 ? 1 : hm;
+      for (int64_t h = 0; h < calls; ++h) {
         const int rc = run(p->part[0], (const double2 *)in + h * sz, (double2 *)out + h * sz,
                            nullptr, nullptr, nullptr, st);
         if (rc != MFFT_SUCCESS) return rc;
@@ -995,6 +1023,10 @@ mfft_plan *mfft_plan_dft_2d(int64_t n0, int64_t n1, int sign, int64_t howmany, u
     mfft_destroy_plan(col);
     mfft_destroy_plan(row);
     return nullptr;
+  }
+  if (col->stages.size() == 1 && howmany > 1 && col->stages[0].reg != 1) {
+    col->outer = howmany;  // one launch over the columns of every array
+    col->outer_in = col->outer_out = n0 * n1;
   }
   auto *p = new mfft_plan;
   p->kind = 3;
