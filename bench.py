@@ -431,7 +431,11 @@ def bench_2d(args, torch, lf):
         p = lf.Plan2D(N, N, lf.FORWARD, hm)
         t = time_steps(torch, lambda: p.call(x, y), args.steps, args.warmup)
         b = 32.0 * N * N * hm
-        per.append({"N": N, "batch": hm, "us": round(t * 1e6, 2), "GB/s": round(b / t / 1e9, 1)})
+        col = lf.Plan(N, lf.FORWARD, N, istride=N, idist=1, ostride=N, odist=1)  # axis-0 part
+        per.append({"N": N, "batch": hm, "us": round(t * 1e6, 2), "GB/s": round(b / t / 1e9, 1),
+                    "axis0_radices": col.radices, "axis0_stages": col.stages,
+                    "axis1_radices": p.radices})
+        del col
         tot_b += b
         tot_t += t
         del x, y

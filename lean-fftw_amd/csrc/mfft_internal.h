@@ -64,7 +64,7 @@ struct PassArgs {
 //
 // Element e of sub-problem g (global index within its transform) of transform h is read
 // from in[h * in_h + g * in_g + e * in_e] and, after the passes, written to
-// out[h * out_h + g * out_g + e * out_e]. Twiddle base kb = g * kb_u. Optional real
+// out[h * out_h + g * out_g + e * out_e]. Twiddle base kb = g * kb_u + h * kb_h. Optional real
 // weights: pre[g * pre_g + e * pre_e] multiplies the input element, post[...] the output.
 struct StageArgs {
   int L, npass, G;
@@ -82,6 +82,7 @@ struct StageArgs {
   int64_t nsub;
   int64_t in_g, in_e, in_h, out_g, out_e, out_h;
   int64_t kb_u;
+  int64_t kb_h;  // twiddle base of workgroup row h: kb = g * kb_u + h * kb_h
   const double *pre;
   const double *post;
   int64_t pre_g, pre_e, post_g, post_e;

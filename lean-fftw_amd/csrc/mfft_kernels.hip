@@ -24,7 +24,7 @@ __global__ __launch_bounds__(MAXT) void stage_kernel(const double2 *in, double2 
   HbmSide hout{out + h * a.out_h + g0 * a.out_g, nullptr, a.out_g, a.out_e,
                a.post ? a.post + g0 * a.post_g : nullptr, a.post_g, a.post_e};
   const double part = run_stage<FWD, RSET, EPT, SQ>(lds, hin, hout, threadIdx.x, blockDim.x, a.G, gcount, a.L,
-                 a.npass, a.pass, a.gfast_in, a.gfast_out, tw, g0, a.kb_u, a.stage_io, a.m_L);
+                 a.npass, a.pass, a.gfast_in, a.gfast_out, tw + h * a.kb_h, g0, a.kb_u, a.stage_io, a.m_L);
   if constexpr (SQ) {
     // deterministic per-workgroup partial of sum |out|^2: a fixed reduction tree
     __shared__ double wsum[MAXT / 64];

@@ -61,6 +61,9 @@ struct mfft_plan {
   // an outer batch of single-stage plans: `outer` repetitions of the whole batch, the k-th
   // at in + k * outer_in, out + k * outer_out (the arrays of a 2-D plan's axis-0 part)
   int64_t outer = 1, outer_in = 0, outer_out = 0;
+  // two-stage plan over a batch of adjacent transforms (columns): workgroups take G
+  // adjacent transforms of one sub-problem index, which grid.y walks (see build_stages)
+  bool columns = false;
 };
 
 namespace {
@@ -345,6 +348,63 @@ void fill_pass(mfft::PassArgs &pa, int R, int64_t nsl, int64_t twns, int64_t kmu
   pa.kmul = kmul;
 }
 
+// Transforms adjacent in memory on both sides (|dist| < |stride|), e.g. the columns of a
+// row-major array.
+bool columns_layout(const mfft_plan *p) {
+  return std::llabs(p->idist) < std::llabs(p->istride) &&
+         std::llabs(p->odist) < std::llabs(p->ostride);
+}
+
+// Two stages over a batch of columns. The passes are those of the row layout's split
+// (stage A: sub-problems q < Q of length P, elements q + Q e; stage B: residue classes
+// k1 < P of length Q, twiddle base k1), but a workgroup takes G adjacent transforms of one
+// sub-problem, so every HBM access is a row of G consecutive complex values: g walks the
+// transforms, grid.y walks q (stage A) or k1 (stage B). The intermediate of a chunk of H
+// transforms is laid out as [q P + e][transform]; the H-dependent strides are set per chunk
+// in run().
+bool build_columns(mfft_plan *p, int split, const std::vector<int64_t> &ns) {
+  const auto &rs = p->radices;
+  const int s = (int)rs.size();
+  const int64_t N = p->n, P = ns[split], Q = N / P, U = std::max<int64_t>(1, p->howmany);
+  const std::vector<int> ra(rs.begin(), rs.begin() + split), rb(rs.begin() + split, rs.end());
+  StageArgs A{}, B{};
+  A.L = (int)P;
+  A.npass = split;
+  strided_tile(A, P, U, p->allow_big, ra);
+  if (p->allow_big) reg_stage(A, ra, U);
+  for (int t = 0; t < split; ++t) fill_pass(A.pass[t], rs[t], ns[t], ns[t], 1, A.L, A.G);
+  A.in_g = p->idist;
+  A.in_e = p->istride * Q;
+  A.in_h = p->istride;
+  A.out_g = 1;
+  A.gfast_in = A.gfast_out = 1;
+  A.pre_e = 1;
+  A.post_e = 1;
+
+  B.L = (int)Q;
+  B.npass = s - split;
+  strided_tile(B, Q, U, p->allow_big, rb);
+  if (p->allow_big) reg_stage(B, rb, U);
+  for (int t = split; t < s; ++t)
+    fill_pass(B.pass[t - split], rs[t], ns[t] / P, ns[t], P, B.L, B.G);
+  B.in_g = 1;
+  B.out_g = p->odist;
+  B.out_e = p->ostride * P;
+  B.out_h = p->ostride;
+  B.kb_h = 1;
+  B.gfast_in = B.gfast_out = 1;
+  B.pre_e = 1;
+  B.post_e = 1;
+  p->columns = true;
+  p->stages = {A, B};
+  p->stage_passes = {split, s - split};
+  const int64_t cmax =
+      std::max<int64_t>(1, std::min<int64_t>(U, chunk_bytes() / (16 * N)));
+  const int64_t nchunk = (U + cmax - 1) / cmax;
+  p->chunk_h = (U + nchunk - 1) / nchunk;
+  return true;
+}
+
 // Build the stage descriptors of a plan whose radices and split are set.
 bool build_stages(mfft_plan *p, int split) {
   const int64_t N = p->n;
@@ -416,6 +476,7 @@ bool build_stages(mfft_plan *p, int split) {
   if (P > mfft::LDS_ELEMS || Q > mfft::LDS_ELEMS || split > mfft::MAX_PASSES ||
       s - split > mfft::MAX_PASSES)
     return false;
+  if (columns_layout(p)) return build_columns(p, split, ns);
   StageArgs A{}, B{};
   A.L = (int)P;
   A.npass = split;
@@ -513,6 +574,26 @@ int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
       const int64_t oc = std::min<int64_t>(65535, p->outer - o0);
       if (mfft::launch_stage(fwd, a, oc, in + o0 * p->outer_in, out + o0 * p->outer_out,
                              p->d_tw, st) != hipSuccess)
+        return MFFT_ERR_HIP;
+    }
+    return MFFT_SUCCESS;
+  }
+  if (p->columns) {
+    if (pre || post || sumsq) return MFFT_ERR_UNSUPPORTED;
+    const int64_t P = p->stages[0].L, Q = p->stages[1].L;
+    for (int64_t h0 = 0; h0 < p->howmany; h0 += p->chunk_h) {
+      const int64_t hc = std::min(p->chunk_h, p->howmany - h0);
+      StageArgs A = p->stages[0], B = p->stages[1];
+      A.nsub = B.nsub = hc;
+      A.out_e = hc;  // intermediate [q P + e][transform]
+      A.out_h = P * hc;
+      B.in_e = P * hc;
+      B.in_h = hc;
+      if (mfft::launch_stage(fwd, A, Q, in + h0 * p->idist, p->d_scratch, p->d_tw, st) !=
+          hipSuccess)
+        return MFFT_ERR_HIP;
+      if (mfft::launch_stage(fwd, B, P, p->d_scratch, out + h0 * p->odist, p->d_tw, st) !=
+          hipSuccess)
         return MFFT_ERR_HIP;
     }
     return MFFT_SUCCESS;
@@ -846,7 +927,9 @@ mfft_plan *plan_many_internal(int64_t n, int sign, int64_t howmany, int64_t istr
     }
   }
   const bool contig = istride == 1 && ostride == 1;
-  if (single_stage(n, allow_big, contig)) {
+  // columns too long for a tile of 8 (128-byte rows) run as two stages of 8-wide rows
+  // rather than one stage of narrower rows
+  if (single_stage(n, allow_big, contig) && !(columns_layout(p) && 8 * n > mfft::BIG_ELEMS)) {
     p->radices = allow_big && contig && reg_size(n) ? reg_radices(n) : choose_radices(n, !allow_big);
   } else {
     int64_t P;

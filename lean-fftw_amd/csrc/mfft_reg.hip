@@ -443,7 +443,7 @@ __global__ __launch_bounds__(L / E * G, 4) void stage_reg_kernel(const double2 *
   }
   const int64_t kbi = (g0 + (gi < gcount ? gi : 0)) * a.kb_u;
   const int64_t kbo = (g0 + (go < gcount ? go : 0)) * a.kb_u;
-  stage_passes<FWD, T, E, SUB, 0, 1, Rs...>(v, lds_s, gi, ti, go, to, a, tw, kbi, kbo);
+  stage_passes<FWD, T, E, SUB, 0, 1, Rs...>(v, lds_s, gi, ti, go, to, a, tw + h * a.kb_h, kbi, kbo);
   int tos = to;
   asm volatile("" : "+v"(tos));  // keep the store addresses out of the pass code
   if (go < gcount) {
@@ -487,6 +487,8 @@ const StageSchedule kStages[] = {
     {256, 32, 3, {8, 8, 4}, false, MFFT_SK(256, 32, 8, 8, 4)},
     {512, 8, 3, {8, 8, 8}, true, MFFT_SK(512, 8, 8, 8, 8)},
     {512, 16, 3, {8, 8, 8}, false, MFFT_SK(512, 16, 8, 8, 8)},
+    {1024, 4, 4, {8, 8, 8, 2}, false, MFFT_SK(1024, 4, 8, 8, 8, 2)},
+    {1024, 8, 4, {8, 8, 8, 2}, true, MFFT_SK(1024, 8, 8, 8, 8, 2)},
 };
 #undef MFFT_SK
 

@@ -144,6 +144,25 @@ def test_strided_plan_many(lf, oracle):
     assert np.all(got[:, N:] == 0)  # nothing written outside the layout
 
 
+COLUMN_CASES = [(64, 40), (128, 19), (256, 33), (512, 9), (1024, 17), (2048, 5), (360, 12),
+                (4096, 24), (8192, 9), (10368, 7), (65536, 3), (1 << 18, 2)]
+
+
+@pytest.mark.parametrize("N,hm", COLUMN_CASES)
+@pytest.mark.parametrize("fwd", [True, False])
+def test_column_layout_bit_exact(lf, oracle, N, hm, fwd):
+    """Columns of a row-major (N, hm) array (istride = ostride = hm, idist = odist = 1):
+    one-stage tiles of adjacent transforms, register stage kernels, and two-stage column
+    plans (N > 2048) with partial tiles of transforms; out of place and in place."""
+    x = oracle.lcg_input(N, seed=N * 3 + hm, howmany=hm)
+    xt = np.ascontiguousarray(x.T)
+    plan = lf.Plan(N, lf.FORWARD if fwd else lf.BACKWARD, hm, istride=hm, idist=1, ostride=hm,
+                   odist=1)
+    ref = np.ascontiguousarray(oracle.ref_transform(fwd, plan.radices, x).T)
+    assert bits_equal(run(lf, plan, xt), ref), (plan.radices, plan.stages)
+    assert bits_equal(run(lf, plan, xt, inplace=True), ref), (plan.radices, plan.stages)
+
+
 def test_guard_words_untouched(lf, oracle):
     """Outputs land exactly on [off, off + howmany*N): guard words around them survive."""
     N, hm = 4096, 3
