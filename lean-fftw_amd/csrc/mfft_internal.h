@@ -83,6 +83,11 @@ struct StageArgs {
   int64_t in_g, in_e, in_h, out_g, out_e, out_h;
   int64_t kb_u;
   int64_t kb_h;  // twiddle base of workgroup row h: kb = g * kb_u + h * kb_h
+  // real-data element maps fused into the first loads / last stores (register kernels only):
+  // io_in 0 complex, 1 real doubles, 2 Hermitian half; io_out 0 complex, 1 real parts,
+  // 2 the first io_n / 2 + 1 elements; io_n the full transform length
+  int io_in, io_out;
+  int64_t io_n;
   const double *pre;
   const double *post;
   int64_t pre_g, pre_e, post_g, post_e;

@@ -47,6 +47,31 @@ __device__ __forceinline__ void st_stream(double2 *p, double2 v) {
   __builtin_nontemporal_store(t, reinterpret_cast<d2v *>(p));
 }
 
+// Real-data element maps fused into a transform's first loads and last stores (contiguous
+// layouts; `off` is the element's index within its transform, `base` the transform's start in
+// elements of the array's own type). Input: io_in 1 reads doubles as (x, 0); 2 reads the
+// Hermitian extension of n/2 + 1 complex values (conj of element n - off above n/2; cached
+// loads, since each half element is read twice). Output: io_out 1 stores real parts as
+// doubles; 2 stores the first n/2 + 1 elements only.
+__device__ __forceinline__ double2 load_io(const StageArgs &a, const double2 *in, int64_t base,
+                                           int64_t off) {
+  if (a.io_in == 1)
+    return make_double2(__builtin_nontemporal_load(reinterpret_cast<const double *>(in) + base + off),
+                        0.0);
+  const bool mirror = off > a.io_n / 2;
+  double2 z = in[base + (mirror ? a.io_n - off : off)];
+  if (mirror) z.y = -z.y;
+  return z;
+}
+
+__device__ __forceinline__ void store_io(const StageArgs &a, double2 *out, int64_t base,
+                                         int64_t off, double2 y) {
+  if (a.io_out == 1)
+    __builtin_nontemporal_store(y.x, reinterpret_cast<double *>(out) + base + off);
+  else if (off <= a.io_n / 2)
+    st_stream(out + base + off, y);
+}
+
 // q = n / d for n * d < 2^32, with m = ceil(2^32 / d) (m = 0 encodes d = 1)
 __device__ __forceinline__ uint32_t fdiv(uint32_t n, uint32_t m) {
   return m ? __umulhi(n, m) : n;

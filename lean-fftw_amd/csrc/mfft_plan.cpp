@@ -615,11 +615,65 @@ int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
   return MFFT_SUCCESS;
 }
 
-// Composite plans: the element maps of mfft_real.hip around the complex parts.
+// Whether every stage of a contiguous plan runs in a register kernel, which can take the
+// real-data element maps into its first loads and last stores (the input maps at 8 elements
+// per thread, or real input into a 16-per-thread schedule that opens with radix 16).
+bool fusable(const mfft_plan *q, int io_in) {
+  if (q->istride != 1 || q->ostride != 1 || q->columns || q->stages.empty()) return false;
+  for (const auto &a : q->stages)
+    if (a.reg == 0) return false;
+  const StageArgs &a0 = q->stages[0];
+  return a0.ept == 8 || (io_in == 1 && a0.reg == 1 && a0.pass[0].R == 16);
+}
+
+// A contiguous complex plan with the real-data maps fused: input real doubles (io_in 1) or
+// Hermitian halves (2), output real parts (io_out 1) or the first n/2 + 1 values (2). The
+// arithmetic is the complex plan's on the embedded / extended batch, so the bits equal the
+// unfused composition's.
+int run_io(const mfft_plan *q, const void *in, void *out, int io_in, int io_out, hipStream_t st) {
+  const int64_t n = q->n, nh = n / 2 + 1;
+  const int64_t din = io_in == 2 ? nh : n, dout = io_out == 2 ? nh : n;  // per transform
+  const size_t bin = io_in == 1 ? sizeof(double) : sizeof(double2);
+  const size_t bout = io_out == 1 ? sizeof(double) : sizeof(double2);
+  const bool fwd = q->sign < 0;
+  if (q->stages.size() == 1) {
+    StageArgs a = q->stages[0];
+    a.io_in = io_in;
+    a.io_out = io_out;
+    a.io_n = n;
+    a.in_g = din;
+    a.out_g = dout;
+    return mfft::launch_stage(fwd, a, 1, (const double2 *)in, (double2 *)out, q->d_tw, st) ==
+                   hipSuccess
+               ? MFFT_SUCCESS
+               : MFFT_ERR_HIP;
+  }
+  for (int64_t h0 = 0; h0 < q->howmany; h0 += q->chunk_h) {
+    const int64_t hc = std::min(q->chunk_h, q->howmany - h0);
+    StageArgs A = q->stages[0], B = q->stages[1];
+    A.io_in = io_in;
+    A.io_n = n;
+    A.in_h = din;
+    B.io_out = io_out;
+    B.io_n = n;
+    B.out_h = dout;
+    const auto *src = (const double2 *)((const char *)in + (size_t)h0 * din * bin);
+    auto *dst = (double2 *)((char *)out + (size_t)h0 * dout * bout);
+    if (mfft::launch_stage(fwd, A, hc, src, q->d_scratch, q->d_tw, st) != hipSuccess ||
+        mfft::launch_stage(fwd, B, hc, q->d_scratch, dst, q->d_tw, st) != hipSuccess)
+      return MFFT_ERR_HIP;
+  }
+  return MFFT_SUCCESS;
+}
+
+// Composite plans: the element maps of mfft_real.hip around the complex parts, or fused
+// into the register kernels' loads and stores where every stage has one.
 int run_composite(const mfft_plan *p, const void *in, void *out, hipStream_t st) {
   if (!in || !out) return MFFT_ERR_INVALID;
   const int64_t hm = p->howmany;
   if (hm == 0) return MFFT_SUCCESS;
+  if (p->kind == 1 && !p->d_work) return run_io(p->part[0], in, out, 1, 2, st);
+  if (p->kind == 2 && !p->d_work) return run_io(p->part[0], in, out, 2, 1, st);
   switch (p->kind) {
     case 1: {  // r2c: embed, forward DFT in place, first n/2 + 1 outputs
       if (mfft::launch_embed_real((const double *)in, p->d_work, p->n, hm, st) != hipSuccess)
@@ -682,7 +736,7 @@ mfft_plan *composite_1d(int kind, int64_t n, int64_t howmany, unsigned flags) {
   p->part[0] = inner;
   p->radices = inner->radices;
   p->stage_passes = inner->stage_passes;
-  if (howmany > 0 &&
+  if (howmany > 0 && !fusable(inner, kind == 1 ? 1 : 2) &&
       hipMalloc(&p->d_work, (size_t)howmany * n * sizeof(double2)) != hipSuccess) {
     mfft_destroy_plan(p);
     return nullptr;

@@ -127,12 +127,27 @@ constexpr int reg_block() {
 }
 
 // Load a transform's columns t + T m (non-temporal), times the input weights.
-template <int T, int E>
-__device__ __forceinline__ void reg_load(double2 (&v)[E], const StageArgs &a, const double2 *src,
-                                         int t, bool valid) {
+template <int T, int E, bool REAL16 = false>
+__device__ __forceinline__ void reg_load(double2 (&v)[E], const StageArgs &a, const double2 *in,
+                                         int64_t base, int t, bool valid) {
+  // 16 elements per thread: real input only, and only in the kernels that take it without
+  // spilling (REAL16); the others run no map (the host does not fuse for them)
+  if ((E == 16 && !REAL16) || a.io_in == 0) {
+    const double2 *src = in + base;
 #pragma unroll
-  for (int m = 0; m < E; ++m)
-    v[m] = valid ? ld_stream(src + (int64_t)(t + T * m) * a.in_e) : make_double2(0.0, 0.0);
+    for (int m = 0; m < E; ++m)
+      v[m] = valid ? ld_stream(src + (int64_t)(t + T * m) * a.in_e) : make_double2(0.0, 0.0);
+  } else if constexpr (E == 16) {
+    const double *src = reinterpret_cast<const double *>(in) + base;
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+      v[m] = make_double2(
+          valid ? __builtin_nontemporal_load(src + (int64_t)(t + T * m) * a.in_e) : 0.0, 0.0);
+  } else {
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+      v[m] = valid ? load_io(a, in, base, (int64_t)(t + T * m) * a.in_e) : make_double2(0.0, 0.0);
+  }
   if (a.pre) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -149,10 +164,18 @@ __device__ __forceinline__ void reg_load(double2 (&v)[E], const StageArgs &a, co
 // computing these addresses before the passes and holding them in registers through them.)
 template <int T, int E>
 __device__ __forceinline__ double reg_store(const double2 (&v)[E], const StageArgs &a,
-                                            double2 *dst, int t, bool valid) {
+                                            double2 *out, int64_t base, int t, bool valid) {
   int ts = t;
   asm volatile("" : "+v"(ts));
+  if (a.io_out != 0) {  // real-data maps: composite plans take no weights or partials
+    if (valid) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) store_io(a, out, base, (int64_t)(ts + T * m) * a.out_e, v[m]);
+    }
+    return 0.0;
+  }
   double sumsq = 0.0;
+  double2 *dst = out + base;
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int e = ts + T * m;
@@ -200,7 +223,7 @@ __global__ __launch_bounds__(reg_block<T>(), 4) void reg_kernel(const double2 *i
   const int64_t h = (int64_t)blockIdx.x * G + g;
   const bool valid = h < a.nsub;
   double2 v[E];
-  reg_load<T, E>(v, a, in + (valid ? h : 0) * a.in_g, t, valid);
+  reg_load<T, E>(v, a, in, (valid ? h : 0) * a.in_g, t, valid);
   for (int p = 0; p < a.npass; ++p) {
     const PassArgs pa = a.pass[p];
     const bool twd = pa.twoff >= 0;
@@ -232,7 +255,7 @@ __global__ __launch_bounds__(reg_block<T>(), 4) void reg_kernel(const double2 *i
     }
 #undef MFFT_REG
   }
-  const double sumsq = reg_store<T, E>(v, a, out + (valid ? h : 0) * a.out_g, t, valid);
+  const double sumsq = reg_store<T, E>(v, a, out, (valid ? h : 0) * a.out_g, t, valid);
   if constexpr (SQ) block_partial<reg_block<T>()>(sumsq, a.sumsq + blockIdx.x);
 }
 
@@ -256,6 +279,11 @@ __device__ __forceinline__ void fixed_passes(double2 (&v)[E], int t, void *lds,
   }
 }
 
+template <int R, int... Rest>
+constexpr int first_radix() {
+  return R;
+}
+
 // Below 2048 points a workgroup runs G = 256 / T transforms, each with its own LDS region,
 // and exchanges whole complex values (LDS is not the limit there); from 2048 on one transform
 // and the real/imaginary exchange.
@@ -272,9 +300,9 @@ __global__ __launch_bounds__(reg_block<T>(), 4) void fixed_kernel(const double2 
   const int64_t h = (int64_t)blockIdx.x * G + g;
   const bool valid = G == 1 || h < a.nsub;  // one transform per workgroup: the grid is the batch
   double2 v[E];
-  reg_load<T, E>(v, a, in + (valid ? h : 0) * a.in_g, t, valid);
+  reg_load<T, E, first_radix<Rs...>() == 16>(v, a, in, (valid ? h : 0) * a.in_g, t, valid);
   fixed_passes<FWD, T, E, HALF, 0, 1, Rs...>(v, t, lds_d, a, tw);
-  const double sumsq = reg_store<T, E>(v, a, out + (valid ? h : 0) * a.out_g, t, valid);
+  const double sumsq = reg_store<T, E>(v, a, out, (valid ? h : 0) * a.out_g, t, valid);
   if constexpr (SQ) block_partial<reg_block<T>()>(sumsq, a.sumsq + blockIdx.x);
 }
 
@@ -427,10 +455,18 @@ __global__ __launch_bounds__(L / E * G, 4) void stage_reg_kernel(const double2 *
   double2 v[E];
   {
     const bool ok = gi < gcount;
-    const double2 *src = in + h * a.in_h + (g0 + (ok ? gi : 0)) * a.in_g;
+    if (a.io_in == 0) {
+      const double2 *src = in + h * a.in_h + (g0 + (ok ? gi : 0)) * a.in_g;
 #pragma unroll
-    for (int m = 0; m < E; ++m)
-      v[m] = ok ? ld_stream(src + (int64_t)(ti + T * m) * a.in_e) : make_double2(0.0, 0.0);
+      for (int m = 0; m < E; ++m)
+        v[m] = ok ? ld_stream(src + (int64_t)(ti + T * m) * a.in_e) : make_double2(0.0, 0.0);
+    } else {
+      const int64_t o = (g0 + (ok ? gi : 0)) * a.in_g;
+#pragma unroll
+      for (int m = 0; m < E; ++m)
+        v[m] = ok ? load_io(a, in, h * a.in_h, o + (int64_t)(ti + T * m) * a.in_e)
+                  : make_double2(0.0, 0.0);
+    }
     if (a.pre) {
       const double *w = a.pre + (g0 + (ok ? gi : 0)) * a.pre_g;
 #pragma unroll
@@ -458,7 +494,10 @@ __global__ __launch_bounds__(L / E * G, 4) void stage_reg_kernel(const double2 *
         y.x = y.x * c;
         y.y = y.y * c;
       }
-      st_stream(dst + (int64_t)e * a.out_e, y);
+      if (a.io_out == 0)
+        st_stream(dst + (int64_t)e * a.out_e, y);
+      else
+        store_io(a, out, h * a.out_h, (g0 + go) * a.out_g + (int64_t)e * a.out_e, y);
     }
   }
 }

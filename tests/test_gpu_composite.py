@@ -52,8 +52,10 @@ def herm_ext(h, n):
                     np.conj(h[:, np.minimum((n - k) % n, n // 2)]))
 
 
-REAL_SIZES = [2, 3, 4, 5, 8, 15, 16, 60, 105, 256, 1000, 1024, 2100, 4096, 10368, 65536,
-              165375, 262144]
+# powers of two from 64 on run with the maps fused into the register kernels (16384 through
+# the embedded batch on the input side); the others through the element-map kernels
+REAL_SIZES = [2, 3, 4, 5, 8, 15, 16, 32, 60, 64, 105, 128, 256, 512, 1000, 1024, 2048, 2100,
+              4096, 8192, 10368, 16384, 32768, 65536, 131072, 165375, 262144]
 
 
 @pytest.mark.parametrize("n", REAL_SIZES)
