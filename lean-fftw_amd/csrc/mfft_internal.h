@@ -88,6 +88,9 @@ struct StageArgs {
   // 2 the first io_n / 2 + 1 elements; io_n the full transform length
   int io_in, io_out;
   int64_t io_n;
+  // 1: ordinary (cached) loads / stores on that side instead of non-temporal ones (set by
+  // the planner where they measured faster: a two-stage intermediate, register-kernel outputs)
+  int cached_in, cached_out;
   const double *pre;
   const double *post;
   int64_t pre_g, pre_e, post_g, post_e;

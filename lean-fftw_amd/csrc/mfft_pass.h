@@ -87,6 +87,7 @@ struct HbmSide {
   int64_t gstride, stride;
   const double *wt;
   int64_t wg, wstride;
+  int cached = 0;  // output side: ordinary stores instead of non-temporal ones
 };
 
 __device__ __forceinline__ int64_t hbm_off(const HbmSide &s, int g) {
@@ -176,8 +177,13 @@ __device__ __forceinline__ double pass_run(double2 *buf, const HbmSide &hin, con
           x[u][r].y = x[u][r].y * v;
         }
       }
+      if (hout.cached) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) st_stream(d + r * st, x[u][r]);
+        for (int r = 0; r < R; ++r) d[r * st] = x[u][r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) st_stream(d + r * st, x[u][r]);
+      }
       if constexpr (SQ) {
 #pragma unroll
         for (int r = 0; r < R; ++r) sumsq = sumsq + (x[u][r].x * x[u][r].x + x[u][r].y * x[u][r].y);
@@ -239,7 +245,10 @@ __device__ __forceinline__ double tile_from_lds(const double2 *buf, const HbmSid
         v.x = v.x * w;
         v.y = v.y * w;
       }
-      st_stream(h.base + f, v);
+      if (h.cached)
+        h.base[f] = v;
+      else
+        st_stream(h.base + f, v);
       if constexpr (SQ) sumsq = sumsq + (v.x * v.x + v.y * v.y);
     }
   }

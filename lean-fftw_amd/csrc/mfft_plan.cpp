@@ -393,6 +393,7 @@ bool build_columns(mfft_plan *p, int split, const std::vector<int64_t> &ns) {
   B.out_h = p->ostride;
   B.kb_h = 1;
   B.gfast_in = B.gfast_out = 1;
+  A.cached_out = 1;  // the intermediate, as in the row layout
   B.pre_e = 1;
   B.post_e = 1;
   p->columns = true;
@@ -466,6 +467,9 @@ bool build_stages(mfft_plan *p, int split) {
       a.threads = T < 256 ? 256 : T;
       a.G = a.threads / T;
       a.stage_io = 0;
+      // ordinary stores of the output measured 2-8% faster than non-temporal ones for these
+      // kernels on MI355X (64..8192 points; neutral at 16 elements per thread, 16384)
+      a.cached_out = E == 8 ? 1 : 0;
     }
     p->stages.push_back(a);
     p->stage_passes.push_back(s);
@@ -514,6 +518,10 @@ bool build_stages(mfft_plan *p, int split) {
   B.gfast_out = 1;
   B.post_g = 1;
   B.post_e = P;
+  // the intermediate is written with ordinary stores: 10-15% faster two-stage plans on MI355X
+  // than non-temporal ones (2^15..2^18 points; reads of it stay non-temporal, which measured
+  // better than ordinary loads)
+  A.cached_out = 1;
   p->stages.push_back(A);
   p->stages.push_back(B);
   p->stage_passes.push_back(split);

@@ -134,9 +134,15 @@ __device__ __forceinline__ void reg_load(double2 (&v)[E], const StageArgs &a, co
   // spilling (REAL16); the others run no map (the host does not fuse for them)
   if ((E == 16 && !REAL16) || a.io_in == 0) {
     const double2 *src = in + base;
+    if (a.cached_in) {
 #pragma unroll
-    for (int m = 0; m < E; ++m)
-      v[m] = valid ? ld_stream(src + (int64_t)(t + T * m) * a.in_e) : make_double2(0.0, 0.0);
+      for (int m = 0; m < E; ++m)
+        v[m] = valid ? src[(int64_t)(t + T * m) * a.in_e] : make_double2(0.0, 0.0);
+    } else {
+#pragma unroll
+      for (int m = 0; m < E; ++m)
+        v[m] = valid ? ld_stream(src + (int64_t)(t + T * m) * a.in_e) : make_double2(0.0, 0.0);
+    }
   } else if constexpr (E == 16) {
     const double *src = reinterpret_cast<const double *>(in) + base;
 #pragma unroll
@@ -185,7 +191,12 @@ __device__ __forceinline__ double reg_store(const double2 (&v)[E], const StageAr
       y.x = y.x * w;
       y.y = y.y * w;
     }
-    if (valid) st_stream(dst + (int64_t)e * a.out_e, y);
+    if (valid) {
+      if (a.cached_out)
+        dst[(int64_t)e * a.out_e] = y;
+      else
+        st_stream(dst + (int64_t)e * a.out_e, y);
+    }
     sumsq = sumsq + (y.x * y.x + y.y * y.y);
   }
   return sumsq;
@@ -459,7 +470,9 @@ __global__ __launch_bounds__(L / E * G, 4) void stage_reg_kernel(const double2 *
       const double2 *src = in + h * a.in_h + (g0 + (ok ? gi : 0)) * a.in_g;
 #pragma unroll
       for (int m = 0; m < E; ++m)
-        v[m] = ok ? ld_stream(src + (int64_t)(ti + T * m) * a.in_e) : make_double2(0.0, 0.0);
+        v[m] = ok ? (a.cached_in ? src[(int64_t)(ti + T * m) * a.in_e]
+                                 : ld_stream(src + (int64_t)(ti + T * m) * a.in_e))
+                  : make_double2(0.0, 0.0);
     } else {
       const int64_t o = (g0 + (ok ? gi : 0)) * a.in_g;
 #pragma unroll
@@ -494,9 +507,12 @@ __global__ __launch_bounds__(L / E * G, 4) void stage_reg_kernel(const double2 *
         y.x = y.x * c;
         y.y = y.y * c;
       }
-      if (a.io_out == 0)
-        st_stream(dst + (int64_t)e * a.out_e, y);
-      else
+      if (a.io_out == 0) {
+        if (a.cached_out)
+          dst[(int64_t)e * a.out_e] = y;
+        else
+          st_stream(dst + (int64_t)e * a.out_e, y);
+      } else
         store_io(a, out, h * a.out_h, (g0 + go) * a.out_g + (int64_t)e * a.out_e, y);
     }
   }
