@@ -57,6 +57,13 @@ typedef struct mfft_ragged mfft_ragged;
 mfft_plan *mfft_plan_many(int64_t n, int sign, int64_t howmany, int64_t istride,
                           int64_t idist, int64_t ostride, int64_t odist, unsigned flags);
 
+/* The same entry point under the name of the reference-side plan record constructor a
+ * caller may already bind (lfa_plan_measure's replacement, Planner.lean:291-302). Planning
+ * never reads or writes caller arrays (MEASURE times candidate schedules on buffers of its
+ * own), so there is no in-place-measure flag. */
+mfft_plan *mfft_plan_create(int64_t n, int sign, int64_t howmany, int64_t istride,
+                            int64_t idist, int64_t ostride, int64_t odist, unsigned flags);
+
 /* Contiguous batch: istride = ostride = 1, idist = odist = n.
  * Replaces lfa_plan_measure(N, fwd, in, out) (with howmany = 1). */
 mfft_plan *mfft_plan_dft_1d(int64_t n, int sign, int64_t howmany, unsigned flags);
@@ -107,6 +114,9 @@ int mfft_execute_host(const mfft_plan *p, const double *in, double *out);
 int mfft_execute_spec(const mfft_plan *p, const double *in, double *out, void *stream);
 
 void mfft_destroy_plan(mfft_plan *p);
+/* Alias of mfft_destroy_plan (replaces the adapter's lean_dec(aplan),
+ * bench-leanfftw.c:293-297). */
+void mfft_plan_destroy(mfft_plan *p);
 
 /* Plan description: the radix sequence R_0..R_{s-1} of the Stockham schedule (the
  * reference's `rs`, Stockham.lean:112) and the number of passes fused into each kernel

@@ -1082,6 +1082,13 @@ int mfft_plan_schedule(int64_t n, int *radices, int max, int *split) {
   return (int)rs.size();
 }
 
+mfft_plan *mfft_plan_create(int64_t n, int sign, int64_t howmany, int64_t istride,
+                            int64_t idist, int64_t ostride, int64_t odist, unsigned flags) {
+  return mfft_plan_many(n, sign, howmany, istride, idist, ostride, odist, flags);
+}
+
+void mfft_plan_destroy(mfft_plan *p) { mfft_destroy_plan(p); }
+
 mfft_plan *mfft_plan_dft_1d(int64_t n, int sign, int64_t howmany, unsigned flags) {
   return mfft_plan_many(n, sign, howmany, 1, n, 1, n, flags);
 }
