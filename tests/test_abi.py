@@ -148,3 +148,37 @@ def test_planner_schedules_are_valid(lf):
             assert n == 16384 and rs == [16, 16, 16, 4], (n, rs)
     with pytest.raises(lf.MfftError):
         lf.schedule(17)
+
+
+def _bench2():
+    import importlib.util
+    path = os.path.join(ROOT, "lean-fftw_amd", "bench2.py")
+    spec = importlib.util.spec_from_file_location("mfft_bench2", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_bench2_problem_strings_and_output_format():
+    """The libbench2-shaped command line: problem grammar, time format and mflops convention,
+    and a speed line the reference's protocol runner parses (tools/bench.py:45-49)."""
+    import re
+
+    b = _bench2()
+    p = b.parse_problem("ocf1024")
+    assert p == {"name": "ocf1024", "inplace": False, "real": False, "forward": True,
+                 "dims": [1024]}
+    assert b.parse_problem("icb64x128")["dims"] == [64, 128]
+    assert b.parse_problem("orb1000")["real"] and not b.parse_problem("orb1000")["forward"]
+    for bad in ("ocf", "xcf8", "irf8", "orf4x4", "ocf0"):
+        with pytest.raises(ValueError):
+            b.parse_problem(bad)
+    assert b.fmt_time(1.25e-6) == "1.25 us" and b.fmt_time(19e-6) == "19.00 us"
+    assert b.fmt_time(3.5e-9) == "3.50 ns" and b.fmt_time(2.0) == "2.00 s"
+    # libbench2's convention, e.g. ocf1024 at 1.25 us -> 40960 "mflops"
+    assert abs(b.mflops(p, 1.25e-6) - 5 * 1024 * 10 / 1.25) < 1e-6
+    line = f"Problem: ocf1024, setup: {b.fmt_time(19e-6)}, time: {b.fmt_time(1.25e-6)}, " \
+           f"``mflops'': {b.mflops(p, 1.25e-6):.3f}"
+    m = re.search(r"setup:\s*([0-9.e+-]+)\s*(ns|us|ms|s),\s*time:\s*([0-9.e+-]+)\s*(ns|us|ms|s)", line)
+    assert m and float(m.group(3)) == 1.25
+    assert b.main(["-s", "bogus"]) == 2
