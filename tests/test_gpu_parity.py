@@ -235,7 +235,13 @@ def test_deterministic_rerun(lf, oracle):
 # ------------------------------------------------------------------ accuracy vs reference
 
 
-@pytest.mark.parametrize("N", [8, 64, 1024, 4096, 16384, 65536, 262144, 6, 1000, 10368, 75600, 165375])
+# every size with a published reference L2: 2^1..2^18 and the 18 cross-checked 13-smooth
+# sizes (results/repro-2026-11-04/xcheck-patient.jsonl), config 3's published ones included
+ACCURACY_SIZES = [2 ** k for k in range(1, 19)] + [6, 9, 12, 15, 18, 24, 36, 80, 108, 210, 504,
+                                                   1000, 1960, 4725, 10368, 27000, 75600, 165375]
+
+
+@pytest.mark.parametrize("N", ACCURACY_SIZES)
 def test_accuracy_within_2x_reference(lf, oracle, N):
     import json
     import os
@@ -243,22 +249,31 @@ def test_accuracy_within_2x_reference(lf, oracle, N):
     from conftest import GOLDEN
 
     acc = json.load(open(os.path.join(GOLDEN, "accuracy_ref.json")))
-    ref = None
-    if str(N) in acc["pow2"]:
-        ref = acc["pow2"][str(N)]["fwd_l2"]
-    elif str(N) in acc["xcheck"]:
-        ref = acc["xcheck"][str(N)]["lean_l2"]
-    assert ref is not None
+    # two published measurements: libbench2 --accuracy (emr-i12, its own inputs) and the
+    # cross-check against long-double FFTW (xcheck); where both exist the bar takes the
+    # larger (at N = 4 libbench2's inputs make the transform exact, 3e-44, the cross-check
+    # reads 5.6e-17)
+    refs = [acc[k][str(N)][f] for k, f in (("pow2", "fwd_l2"), ("xcheck", "lean_l2"))
+            if str(N) in acc[k]]
+    assert refs
+    ref = max(refs)
     rounds = 4 if N >= 65536 else 10
     x = oracle.lcg_input(N, seed=4242 + N, howmany=rounds)
     plan = lf.Plan(N, lf.FORWARD, rounds)
     got = run(lf, plan, x)
     err = oracle.rel_l2(True, x, got)
+    log = os.environ.get("MFFT_ACCURACY_LOG")
+    if log:  # one JSON line per size, for the per-size table of the design notes
+        with open(log, "a") as f:
+            f.write(json.dumps({"N": N, "radices": plan.radices, "l2_mean": float(err.mean()),
+                                "l2_max": float(err.max()), "reference_l2": ref}) + "\n")
     # libbench2's --accuracy averages the relative L1/L2 errors over its rounds (and takes
     # the max of L-inf); the bar is 2x the reference's published figure for the mean, with
     # every single round within 3x.
-    assert err.mean() <= 2 * ref, (N, err.mean(), ref)
-    assert err.max() <= 3 * ref, (N, err.max(), ref)
+    # (N = 2 and 4 are exact in binary64: the reference reports 0 and 3e-44, and what is left
+    # here is the binary128 comparison's own rounding, so the bar has a floor of 1e-32)
+    assert err.mean() <= max(2 * ref, 1e-32), (N, err.mean(), ref)
+    assert err.max() <= max(3 * ref, 1e-32), (N, err.max(), ref)
 
 
 # ------------------------------------------------------------------ full-size properties
