@@ -20,7 +20,7 @@ __global__ __launch_bounds__(MAXT) void stage_kernel(const double2 *in, double2 
   const int64_t h = blockIdx.y;
   const int gcount = (int)(a.nsub - g0 < a.G ? a.nsub - g0 : a.G);
   HbmSide hin{const_cast<double2 *>(in) + h * a.in_h + g0 * a.in_g, nullptr, a.in_g, a.in_e,
-              a.pre ? a.pre + g0 * a.pre_g : nullptr, a.pre_g, a.pre_e};
+              a.pre ? a.pre + g0 * a.pre_g : nullptr, a.pre_g, a.pre_e, a.cached_in};
   HbmSide hout{out + h * a.out_h + g0 * a.out_g, nullptr, a.out_g, a.out_e,
                a.post ? a.post + g0 * a.post_g : nullptr, a.post_g, a.post_e, a.cached_out};
   const double part = run_stage<FWD, RSET, EPT, SQ>(lds, hin, hout, threadIdx.x, blockDim.x, a.G, gcount, a.L,

@@ -87,7 +87,7 @@ struct HbmSide {
   int64_t gstride, stride;
   const double *wt;
   int64_t wg, wstride;
-  int cached = 0;  // output side: ordinary stores instead of non-temporal ones
+  int cached = 0;  // ordinary loads / stores on this side instead of non-temporal ones
 };
 
 __device__ __forceinline__ int64_t hbm_off(const HbmSide &s, int g) {
@@ -130,8 +130,13 @@ __device__ __forceinline__ double pass_run(double2 *buf, const HbmSide &hin, con
       if constexpr (SRC_HBM) {
         const double2 *s = hin.base + hbm_off(hin, g) + (int64_t)j * hin.stride;
         const int64_t st = (int64_t)LR * hin.stride;
+        if (hin.cached) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) x[u][r] = ld_stream(s + r * st);
+          for (int r = 0; r < R; ++r) x[u][r] = s[r * st];
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) x[u][r] = ld_stream(s + r * st);
+        }
         if (hin.wt) {
           const double *w = hin.wt + (int64_t)g * hin.wg + (int64_t)j * hin.wstride;
           const int64_t ws = (int64_t)LR * hin.wstride;

@@ -470,6 +470,12 @@ bool build_stages(mfft_plan *p, int split) {
       // ordinary stores of the output measured 2-8% faster than non-temporal ones for these
       // kernels on MI355X (64..8192 points; neutral at 16 elements per thread, 16384)
       a.cached_out = E == 8 ? 1 : 0;
+    } else if (a.reg == 0 && !a.gfast_in && !a.stage_io) {
+      // the LDS-tile kernel's first pass reads each butterfly's inputs in runs of a few
+      // elements, so several load instructions share a cache line: ordinary loads keep it
+      // in L2 for them (MI355X: 60 points 117 -> 91 us, 360 and 1000 points 110 -> 101-106,
+      // 32 points 94 -> 86 per 256 MB batch)
+      a.cached_in = 1;
     }
     p->stages.push_back(a);
     p->stage_passes.push_back(s);
@@ -519,9 +525,11 @@ bool build_stages(mfft_plan *p, int split) {
   B.post_g = 1;
   B.post_e = P;
   // the intermediate is written with ordinary stores: 10-15% faster two-stage plans on MI355X
-  // than non-temporal ones (2^15..2^18 points; reads of it stay non-temporal, which measured
-  // better than ordinary loads)
+  // than non-temporal ones (2^15..2^18 points). The register stage kernel reads it back
+  // non-temporally (ordinary loads measured slower there); the LDS-tile stage kernel with
+  // ordinary loads (10368 points 257 -> 225 us, 165375 points 410 -> 357 per 256 MB batch)
   A.cached_out = 1;
+  B.cached_in = B.reg == 0 ? 1 : 0;
   p->stages.push_back(A);
   p->stages.push_back(B);
   p->stage_passes.push_back(split);
