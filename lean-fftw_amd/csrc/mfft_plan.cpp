@@ -997,9 +997,10 @@ mfft_plan *plan_many_internal(int64_t n, int sign, int64_t howmany, int64_t istr
     }
   }
   const bool contig = istride == 1 && ostride == 1;
-  // columns too long for a tile of 8 (128-byte rows) run as two stages of 8-wide rows
-  // rather than one stage of narrower rows
-  if (single_stage(n, allow_big, contig) && !(columns_layout(p) && 8 * n > mfft::BIG_ELEMS)) {
+  // columns too long for a tile of 4 (64-byte rows) run as two stages of wider rows rather
+  // than one stage of narrower rows (MI355X, 2-D columns: 2048 points one stage of 4-wide
+  // rows 311 us against 340 for two stages; 4096 points two stages 274-304 us against 868)
+  if (single_stage(n, allow_big, contig) && !(columns_layout(p) && 4 * n > mfft::BIG_ELEMS)) {
     p->radices = allow_big && contig && reg_size(n) ? reg_radices(n) : choose_radices(n, !allow_big);
   } else {
     int64_t P;
