@@ -30,6 +30,7 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]
 SOURCES = [
     ("mfft_kernels.hip", True),
     ("mfft_reg.hip", True),
+    ("mfft_tile.hip", True),
     ("mfft_real.hip", True),
     ("mfft_ragged.hip", True),
     ("mfft_plan.cpp", False),

@@ -97,6 +97,14 @@ struct StageArgs {
   // optional: each workgroup writes the sum of |out|^2 of its stores to
   // sumsq[blockIdx.y * gridDim.x + blockIdx.x] (last stage of a plan only)
   double *sumsq;
+  // 1 + index of a fixed-schedule tile kernel (mfft_tile.hip) for this stage, 0 for none; it
+  // runs the stage when no weights, partials or element maps are asked for
+  int tile;
+  // 1: grid.x is padded to a multiple of 8 and workgroups are remapped so that each of the 8
+  // XCDs (which take workgroups round-robin in launch order) runs one contiguous eighth of the
+  // sub-problems of every row: stages whose twiddles depend on the sub-problem then read one
+  // eighth of the table per XCD's L2
+  int xcd;
 };
 
 // A size class of a ragged batch: the stages of one plan (at most two), with pass
@@ -136,6 +144,13 @@ hipError_t launch_herm_ext(const double2 *in, double2 *work, int64_t n, int64_t 
                            hipStream_t st);
 hipError_t launch_real_part(const double2 *work, double *out, int64_t n, int64_t howmany,
                             hipStream_t st);
+
+// Fixed-schedule tile kernels (mfft_tile.hip): the kernel for a stage whose shape, radices,
+// lane orders and tile copies match one (1 + its index, 0 if none), and its launch.
+int tile_lookup(const StageArgs &a);
+int xcd_remap(const StageArgs &a);
+hipError_t launch_tile(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,
+                       double2 *out, const double2 *tw, hipStream_t stream);
 
 // Launch one stage over `ntrans` transforms; returns a hipError_t.
 hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,

@@ -67,6 +67,8 @@ hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const doub
       (a.reg == 0 || (a.reg == 2 && a.sumsq) ||
        (a.io_in && a.ept != 8 && !(a.io_in == 1 && a.reg == 1 && a.pass[0].R == 16))))
     return hipErrorInvalidValue;
+  if (a.tile > 0 && !a.pre && !a.post && !a.sumsq && !a.io_in && !a.io_out)
+    return launch_tile(fwd, a, ntrans, in, out, tw, stream);
   if (a.reg == 1) return ntrans == 1 ? launch_reg(fwd, a, in, out, tw, stream) : hipErrorInvalidValue;
   if (a.reg == 2 && !a.sumsq) return launch_reg_stage(fwd, a, ntrans, in, out, tw, stream);
   const int64_t nx = (a.nsub + a.G - 1) / a.G;

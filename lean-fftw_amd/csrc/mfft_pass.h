@@ -77,6 +77,24 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, uint32_t m) {
   return m ? __umulhi(n, m) : n;
 }
 
+// Workgroup coordinates of a stage launch (sub-problem tile bx, row by). With a.xcd the grid's
+// x extent is a multiple of 8 and linear workgroup id i (which runs on XCD i mod 8) takes tile
+// (i mod 8) * c + (i / 8) mod c of row (i / 8) / c, c = gridDim.x / 8: a bijection of the grid
+// in which every XCD keeps to its own eighth of the tiles. Returns false for the padding
+// workgroups past the last tile (the whole workgroup returns before any barrier).
+__device__ __forceinline__ bool stage_block(int xcd, int64_t nsub, int G, int &bx, int &by) {
+  bx = blockIdx.x;
+  by = blockIdx.y;
+  if (xcd) {
+    const uint32_t c = gridDim.x >> 3;
+    const uint32_t lin = blockIdx.x + blockIdx.y * gridDim.x;
+    const uint32_t idx = lin >> 3;
+    bx = (int)((lin & 7u) * c + idx % c);
+    by = (int)(idx / c);
+  }
+  return (int64_t)bx * G < nsub;
+}
+
 // Where a pass reads or writes in HBM: element e of sub-problem g of the tile is
 // base[off(g) + e * stride], off(g) = list ? list[g] : g * gstride, optionally scaled by a
 // real weight wt[g * wg + e * wstride] (weights are shared across the batch). With SQ the

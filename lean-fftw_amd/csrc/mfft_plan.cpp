@@ -460,6 +460,9 @@ bool build_stages(mfft_plan *p, int split) {
                   p->idist == N && p->odist == N)
                      ? 1
                      : 0;
+    // the staged tile is read with ordinary loads (MI355X, tile kernels, 256 MB batches of
+    // 2..16 points: 87-90 us with non-temporal loads, 80-82 with ordinary ones)
+    if (a.stage_io) a.cached_in = 1;
     if (reg) {  // T = N / E threads per transform, 256 / T transforms per workgroup if T < 256
       const int E = mfft::reg_elems(N), T = (int)(N / E);
       a.reg = 1;
@@ -549,6 +552,9 @@ mfft_plan *finish(mfft_plan *p, int split) {
     mfft_destroy_plan(p);
     return nullptr;
   }
+  // stages with a fixed-schedule tile kernel run it (column plans size their stages per
+  // chunk at run time and keep the stage kernel)
+  for (auto &st : p->stages) st.tile = p->columns ? 0 : mfft::tile_lookup(st);
   if (p->stages.size() > 1) {
     if (hipMalloc(&p->d_scratch, (size_t)p->chunk_h * p->n * sizeof(double2)) != hipSuccess) {
       mfft_destroy_plan(p);

@@ -1,0 +1,51 @@
+"""GPU parity of the fixed-schedule LDS-tile kernels (csrc/mfft_tile.hip).
+
+Default plans whose stages have a compile-time tile schedule run these kernels once the batch
+fills a tile, so each case takes a batch of at least one full tile plus a partial last one
+(for two-stage plans the stages' sub-problem counts Q and P do that). Bar: bit-exact against
+the oracle's refTransform for the plan's radices, both directions, out of place and in place.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = pytest.mark.gpu
+
+if not gpu_available():  # pragma: no cover - CPU-only containers
+    pytest.skip("needs a GPU", allow_module_level=True)
+
+import torch  # noqa: E402
+
+DEV = "cuda:0"
+
+# (N, batch): single-stage sizes with tiles of G transforms (2: 1024, 4: 512, 8: 256,
+# 16: 128, 6: 341, 32: 64, 60: 34, 360: 5, 1000: 2, 2100: 1) and two-stage sizes whose
+# stages have tile schedules (10368 = 96 x 108, 75600 = 270 x 280, 165375 = 375 x 441); 64 ...
+# 2048 points run the tile kernels in place of the register-resident kernel, 4096 and the
+# two-stage power-of-two sizes keep the register kernels
+TILE_CASES = [(2, 1537), (4, 769), (8, 385), (16, 193), (6, 513), (32, 97), (60, 52),
+              (360, 8), (1000, 3), (2100, 3), (10368, 2), (75600, 1), (165375, 1),
+              (64, 49), (128, 25), (256, 13), (512, 7), (1024, 3), (2048, 2), (4096, 2),
+              (32768, 1), (65536, 1), (131072, 1), (262144, 1)]
+
+
+def bits_equal(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint64),
+                          np.ascontiguousarray(b).view(np.uint64))
+
+
+@pytest.mark.parametrize("N,hm", TILE_CASES)
+@pytest.mark.parametrize("fwd", [True, False])
+def test_tile_kernels_bit_exact(lf, oracle, N, hm, fwd):
+    x = oracle.lcg_input(N, seed=0x7E1 ^ N ^ (fwd << 33), howmany=hm)
+    plan = lf.Plan(N, lf.FORWARD if fwd else lf.BACKWARD, hm)
+    ref = oracle.ref_transform(fwd, plan.radices, x)
+    xd = torch.from_numpy(np.ascontiguousarray(x)).to(DEV)
+    out = torch.empty_like(xd)
+    plan.call(xd, out)
+    plan.call(xd, xd)  # in place
+    torch.cuda.synchronize()
+    assert bits_equal(out.cpu().numpy(), ref), (N, plan.radices, plan.stages, "out of place")
+    assert bits_equal(xd.cpu().numpy(), ref), (N, plan.radices, plan.stages, "in place")
