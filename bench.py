@@ -172,7 +172,7 @@ def bench_sweep(args, torch, lf):
                    "planner": "measure" if args.measure else "estimate"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "the FFT kernels of one sweep step (stage_kernel, fixed_kernel "
+                     "kernel": "the FFT kernels of one sweep step (tile_kernel, fixed_kernel "
                                "and stage_reg_kernel launches, 22 per step): algorithmic 32N "
                                "bytes per transform over "
                                "the summed per-size HIP-event time; traffic = HBM bytes per step "
