@@ -552,8 +552,8 @@ mfft_plan *finish(mfft_plan *p, int split) {
     mfft_destroy_plan(p);
     return nullptr;
   }
-  // stages with a fixed-schedule tile kernel run it (column plans size their stages per
-  // chunk at run time and keep the stage kernel)
+  // stages with a fixed-schedule tile kernel run it (two-stage column plans size their stages
+  // per chunk and look theirs up at run time)
   for (auto &st : p->stages) st.tile = p->columns ? 0 : mfft::tile_lookup(st);
   if (p->stages.size() > 1) {
     if (hipMalloc(&p->d_scratch, (size_t)p->chunk_h * p->n * sizeof(double2)) != hipSuccess) {
@@ -611,6 +611,8 @@ int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
       A.out_h = P * hc;
       B.in_e = P * hc;
       B.in_h = hc;
+      A.tile = mfft::tile_lookup(A);  // the chunk fixes the tile count
+      B.tile = mfft::tile_lookup(B);
       if (mfft::launch_stage(fwd, A, Q, in + h0 * p->idist, p->d_scratch, p->d_tw, st) !=
           hipSuccess)
         return MFFT_ERR_HIP;
@@ -1005,7 +1007,8 @@ mfft_plan *plan_many_internal(int64_t n, int sign, int64_t howmany, int64_t istr
   const bool contig = istride == 1 && ostride == 1;
   // columns too long for a tile of 4 (64-byte rows) run as two stages of wider rows rather
   // than one stage of narrower rows (MI355X, 2-D columns: 2048 points one stage of 4-wide
-  // rows 311 us against 340 for two stages; 4096 points two stages 274-304 us against 868)
+  // rows 311 us against 340 for two stages, and 325 against 332 with the tile kernels; 4096
+  // points two stages 274-304 us against 868)
   if (single_stage(n, allow_big, contig) && !(columns_layout(p) && 4 * n > mfft::BIG_ELEMS)) {
     p->radices = allow_big && contig && reg_size(n) ? reg_radices(n) : choose_radices(n, !allow_big);
   } else {

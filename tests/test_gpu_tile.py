@@ -49,3 +49,24 @@ def test_tile_kernels_bit_exact(lf, oracle, N, hm, fwd):
     torch.cuda.synchronize()
     assert bits_equal(out.cpu().numpy(), ref), (N, plan.radices, plan.stages, "out of place")
     assert bits_equal(xd.cpu().numpy(), ref), (N, plan.radices, plan.stages, "in place")
+
+
+# columns of a row-major (N, hm) array: single-stage tiles of 32 adjacent transforms of 64
+# points, and both stages (64 x 64) of a two-stage column plan of 4096 points
+COLUMN_TILE_CASES = [(64, 40), (4096, 40)]
+
+
+@pytest.mark.parametrize("N,hm", COLUMN_TILE_CASES)
+@pytest.mark.parametrize("fwd", [True, False])
+def test_column_tile_kernels_bit_exact(lf, oracle, N, hm, fwd):
+    x = oracle.lcg_input(N, seed=0xC01 + N + hm, howmany=hm)
+    xt = torch.from_numpy(np.ascontiguousarray(x.T)).to(DEV)
+    plan = lf.Plan(N, lf.FORWARD if fwd else lf.BACKWARD, hm, istride=hm, idist=1, ostride=hm,
+                   odist=1)
+    ref = np.ascontiguousarray(oracle.ref_transform(fwd, plan.radices, x).T)
+    out = torch.empty_like(xt)
+    plan.call(xt, out)
+    plan.call(xt, xt)  # in place
+    torch.cuda.synchronize()
+    assert bits_equal(out.cpu().numpy(), ref), (N, plan.radices, plan.stages, "out of place")
+    assert bits_equal(xt.cpu().numpy(), ref), (N, plan.radices, plan.stages, "in place")
