@@ -112,12 +112,15 @@ __device__ __forceinline__ int64_t hbm_off(const HbmSide &s, int g) {
   return s.list ? s.list[g] : (int64_t)g * s.gstride;
 }
 
-template <bool FWD, int R, bool SRC_HBM, bool DST_HBM, int EPT, bool SQ>
-__device__ __forceinline__ double pass_run(double2 *buf, const HbmSide &hin, const HbmSide &hout,
-                                         int tid, int nthreads, int G, int gcount, int L,
-                                         const PassArgs &pa, bool gfast,
-                                         const double2 *__restrict__ tw, int64_t kb0,
-                                         int64_t kb_u) {
+// One pass; src_hbm / dst_hbm: the inputs come from HBM / the outputs go to HBM (else the LDS
+// buffer). pass_run fixes them at compile time; the ragged kernel passes them at run time, so
+// that it holds one copy of each radix's pass instead of four.
+template <bool FWD, int R, int EPT, bool SQ>
+__device__ __forceinline__ double pass_body(double2 *buf, const HbmSide &hin, const HbmSide &hout,
+                                          int tid, int nthreads, int G, int gcount, int L,
+                                          const PassArgs &pa, bool gfast,
+                                          const double2 *__restrict__ tw, int64_t kb0,
+                                          int64_t kb_u, const bool SRC_HBM, const bool DST_HBM) {
   // every butterfly of the pass, over all threads: a thread holds NB of them in registers
   // (reads, then barrier, then writes), so the pass runs in place in one LDS buffer. A
   // workgroup holds at most EPT * nthreads elements.
@@ -145,7 +148,7 @@ __device__ __forceinline__ double pass_run(double2 *buf, const HbmSide &hin, con
     js[u] = j;
     ok[u] = b < nb_total && g < gcount;
     if (ok[u]) {
-      if constexpr (SRC_HBM) {
+      if (SRC_HBM) {
         const double2 *s = hin.base + hbm_off(hin, g) + (int64_t)j * hin.stride;
         const int64_t st = (int64_t)LR * hin.stride;
         if (hin.cached) {
@@ -172,7 +175,7 @@ __device__ __forceinline__ double pass_run(double2 *buf, const HbmSide &hin, con
       }
     }
   }
-  if constexpr (!SRC_HBM && !DST_HBM) __syncthreads();  // all reads before the in-place writes
+  if (!SRC_HBM && !DST_HBM) __syncthreads();  // all reads before the in-place writes
   double sumsq = 0.0;  // sum of |output|^2 over this thread's HBM stores, when requested
 #pragma unroll
   for (int u = 0; u < NB; ++u) {
@@ -187,7 +190,7 @@ __device__ __forceinline__ double pass_run(double2 *buf, const HbmSide &hin, con
     }
     Bfly<FWD, R, false>::run(x[u]);
     const int e0 = bb * pa.nsl * R + k;
-    if constexpr (DST_HBM) {
+    if (DST_HBM) {
       double2 *d = hout.base + hbm_off(hout, g) + (int64_t)e0 * hout.stride;
       const int64_t st = (int64_t)pa.nsl * hout.stride;
       if (hout.wt) {
@@ -217,8 +220,18 @@ __device__ __forceinline__ double pass_run(double2 *buf, const HbmSide &hin, con
       for (int r = 0; r < R; ++r) d[lpad(e0 + r * pa.nsl)] = x[u][r];
     }
   }
-  if constexpr (!DST_HBM) __syncthreads();
+  if (!DST_HBM) __syncthreads();
   return sumsq;
+}
+
+template <bool FWD, int R, bool SRC_HBM, bool DST_HBM, int EPT, bool SQ>
+__device__ __forceinline__ double pass_run(double2 *buf, const HbmSide &hin, const HbmSide &hout,
+                                         int tid, int nthreads, int G, int gcount, int L,
+                                         const PassArgs &pa, bool gfast,
+                                         const double2 *__restrict__ tw, int64_t kb0,
+                                         int64_t kb_u) {
+  return pass_body<FWD, R, EPT, SQ>(buf, hin, hout, tid, nthreads, G, gcount, L, pa, gfast, tw,
+                                    kb0, kb_u, SRC_HBM, DST_HBM);
 }
 
 // Coalesced copy of a tile of contiguous sub-problems between HBM and LDS: consecutive lanes
@@ -314,6 +327,54 @@ __device__ __forceinline__ double run_stage(double2 *buf, const HbmSide &hin, co
       pass_run<FWD, RR, false, false, EPT, SQ>(buf, hin, hout, tid, nthreads, G, gcount, L, \
         pa, gfast, tw,  \
                                      kb0, kb_u);                                    \
+    break;
+    switch (pa.R) {
+      MFFT_CASE(2)
+      MFFT_CASE(3)
+      MFFT_CASE(4)
+      MFFT_CASE(5)
+      MFFT_CASE(6)
+      MFFT_CASE(7)
+      MFFT_CASE(8)
+      MFFT_CASE(9)
+      MFFT_CASE(10)
+      MFFT_CASE(11)
+      MFFT_CASE(12)
+      MFFT_CASE(13)
+      MFFT_CASE(14)
+      MFFT_CASE(15)
+      MFFT_CASE(16)
+      default:
+        break;
+    }
+#undef MFFT_CASE
+  }
+  if (stage_io) sumsq = tile_from_lds<EPT, SQ>(buf, hout, tid, nthreads, gcount, L, m_L);
+  return sumsq;
+}
+
+// All passes of a stage as run_stage, with one pass body per radix whose HBM / LDS sides are
+// chosen at run time (a quarter of the code, and fewer registers, than a copy per side
+// combination).
+template <bool FWD, int RSET, int EPT, bool SQ = false>
+__device__ __forceinline__ double run_stage_compact(double2 *buf, const HbmSide &hin,
+                                                   const HbmSide &hout, int tid, int nthreads,
+                                                   int G, int gcount, int L, int npass,
+                                                   const PassArgs *pas, int gfast_in,
+                                                   int gfast_out, const double2 *__restrict__ tw,
+                                                   int64_t kb0, int64_t kb_u, int stage_io = 0,
+                                                   uint32_t m_L = 0) {
+  double sumsq = 0.0;
+  if (stage_io) tile_to_lds<EPT>(buf, hin, tid, nthreads, gcount, L, m_L);
+  for (int p = 0; p < npass; ++p) {
+    const PassArgs pa = pas[p];
+    const bool first = p == 0 && !stage_io, last = p == npass - 1 && !stage_io;
+    const bool gfast = first ? gfast_in : (last ? gfast_out : false);
+#define MFFT_CASE(RR)                                                                          \
+  case RR:                                                                                     \
+    if (in_set(RSET, RR))                                                                      \
+      sumsq = pass_body<FWD, RR, EPT, SQ>(buf, hin, hout, tid, nthreads, G, gcount, L, pa,     \
+                                          gfast, tw, kb0, kb_u, first, last);                  \
     break;
     switch (pa.R) {
       MFFT_CASE(2)
