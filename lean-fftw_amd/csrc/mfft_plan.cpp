@@ -68,6 +68,10 @@ struct mfft_plan {
 
 namespace {
 
+// planner flag of the complex parts of real-data plans: register-kernel schedules into whose
+// loads and stores the real-data maps fuse (not part of the public flags)
+constexpr unsigned kFuseRadices = 1u << 30;
+
 std::atomic<size_t> g_threads{1};
 std::mutex g_wisdom_mu;
 std::string g_wisdom_path;
@@ -181,9 +185,18 @@ std::vector<int> choose_radices(int64_t L, bool mid_only = false) {
 
 // Radix sequence of a register-resident size: radices up to 8 (the fixed schedules of
 // mfft_reg.hip for 2048 ... 8192), and 16 x 16 x 16 x 4 at 16384 (a fixed schedule with 16
-// elements per thread: 125 us per 256 MB batch against 180 for radix 4 only).
-std::vector<int> reg_radices(int64_t n) {
+// elements per thread: 125 us per 256 MB batch against 180 for radix 4 only). Plain complex
+// plans (`fuse` false) take radix 16 where it measured faster on MI355X (256 MB batches):
+// 1024 = 16 x 8 x 8 in a tile kernel (78 us against 85 for 8 x 8 x 8 x 2), 8192 =
+// 16 x 16 x 8 x 4 at 16 elements per thread (98-101 against 106-110). (4096 = 16^3 at 256
+// threads per transform measured 85-92 against 92, but it halves the threads of a batch and
+// its weighted kernel took 34 us against 19 on a 512-transform batch.) The parts of
+// real-data plans (`fuse`) keep the 8-per-thread schedules, into whose loads and stores the
+// real-data maps fuse.
+std::vector<int> reg_radices(int64_t n, bool fuse = false) {
   if (n == 16384) return {16, 16, 16, 4};
+  if (!fuse && n == 1024) return {16, 8, 8};
+  if (!fuse && n == 8192) return {16, 16, 8, 4};
   std::vector<int> rs;
   double c;
   best_radices(n, any_radix, rs, c, 8);
@@ -750,7 +763,7 @@ void composite_bytes(const mfft_plan *p, size_t &in_b, size_t &out_b) {
 mfft_plan *composite_1d(int kind, int64_t n, int64_t howmany, unsigned flags) {
   if (howmany < 0) return nullptr;
   mfft_plan *inner = mfft_plan_dft_1d(n, kind == 1 ? MFFT_FORWARD : MFFT_BACKWARD,
-                                      std::max<int64_t>(howmany, 1), flags);
+                                      std::max<int64_t>(howmany, 1), flags | kFuseRadices);
   if (!inner) return nullptr;
   auto *p = new mfft_plan;
   p->kind = kind;
@@ -1010,7 +1023,8 @@ mfft_plan *plan_many_internal(int64_t n, int sign, int64_t howmany, int64_t istr
   // rows 311 us against 340 for two stages, and 325 against 332 with the tile kernels; 4096
   // points two stages 274-304 us against 868)
   if (single_stage(n, allow_big, contig) && !(columns_layout(p) && 4 * n > mfft::BIG_ELEMS)) {
-    p->radices = allow_big && contig && reg_size(n) ? reg_radices(n) : choose_radices(n, !allow_big);
+    p->radices = allow_big && contig && reg_size(n) ? reg_radices(n, (flags & kFuseRadices) != 0)
+                                                    : choose_radices(n, !allow_big);
   } else {
     int64_t P;
     if (!choose_split(n, P)) {

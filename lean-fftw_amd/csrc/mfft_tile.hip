@@ -208,7 +208,8 @@ const TileSchedule kTiles[] = {
     MFFT_T(6, 1, RS_(6), 0, 0, 1, 256, 256, 6),
     // single-stage contiguous batches read butterfly by butterfly; from 64 points on in place
     // of the register-resident kernel (MI355X, 256 MB batches: 64 ... 1024 points 78-83 us
-    // against 88-98, 2048 points 82 against 90; the register kernel stays ahead from 4096)
+    // against 88-98, 2048 points 82 against 90; radix 16 measured 1-2 us slower at 256, 512
+// and 2048 points, 12 slower as 16 x 16 x 4 at 1024, 6 faster as 16 x 8 x 8)
     MFFT_T(32, 2, RS_(8, 4), 0, 0, 0, 64, 256, 8, 4),
     MFFT_T(64, 2, RS_(8, 8), 0, 0, 0, 32, 256, 8, 8),
     MFFT_T(128, 3, RS_(8, 8, 2), 0, 0, 0, 16, 256, 8, 8, 2),
@@ -216,6 +217,10 @@ const TileSchedule kTiles[] = {
     MFFT_T(512, 3, RS_(8, 8, 8), 0, 0, 0, 4, 256, 8, 8, 8),
     MFFT_T(1024, 4, RS_(8, 8, 8, 2), 0, 0, 0, 2, 256, 8, 8, 8, 2),
     MFFT_T(2048, 4, RS_(8, 8, 8, 4), 0, 0, 0, 1, 256, 8, 8, 8, 4),
+    // the default plain schedule of 1024 points (radix 16: one LDS pass fewer); a single
+    // transform takes a tile of one
+    MFFT_T(1024, 3, RS_(16, 8, 8), 0, 0, 0, 2, 256, 16, 8, 8),
+    MFFT_T(1024, 3, RS_(16, 8, 8), 0, 0, 0, 1, 256, 16, 8, 8),
     MFFT_T(60, 2, RS_(12, 5), 0, 0, 0, 34, 256, 12, 5),
     MFFT_T(360, 3, RS_(12, 10, 3), 0, 0, 0, 5, 256, 12, 10, 3),
     MFFT_T(1000, 3, RS_(10, 10, 10), 0, 0, 0, 2, 256, 10, 10, 10),

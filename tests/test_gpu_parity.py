@@ -109,7 +109,8 @@ def test_stage_splits_bit_exact(lf, oracle, N, rs, split):
 @pytest.mark.parametrize("N,rs", [(2048, [8, 8, 8, 4]), (4096, [8, 8, 8, 8]), (8192, [8, 8, 8, 8, 2]),
                                   (8192, [8, 4, 4, 4, 4, 4]), (16384, [16, 16, 16, 4]),
                                   (16384, [8, 8, 8, 8, 4]), (16384, [4, 4, 4, 4, 4, 4, 4]),
-                                  (4096, [4, 8, 2, 8, 8]), (1024, [2, 4, 8, 4, 4])])
+                                  (4096, [4, 8, 2, 8, 8]), (1024, [2, 4, 8, 4, 4]),
+                                  (8192, [16, 16, 8, 4])])
 def test_register_schedules_bit_exact(lf, oracle, N, rs):
     """The register-resident kernel's fixed (compile-time) schedules and its run-time
     dispatched ones, both directions, in and out of place."""

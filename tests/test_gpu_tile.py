@@ -39,8 +39,22 @@ def bits_equal(a, b):
 @pytest.mark.parametrize("N,hm", TILE_CASES)
 @pytest.mark.parametrize("fwd", [True, False])
 def test_tile_kernels_bit_exact(lf, oracle, N, hm, fwd):
+    run_case(lf, oracle, N, hm, fwd, None)
+
+
+# the radix-16 tile schedules with explicit radices: a tile of two and a single transform
+RADIX16_CASES = [(1024, 3, [16, 8, 8]), (1024, 1, [16, 8, 8])]
+
+
+@pytest.mark.parametrize("N,hm,rs", RADIX16_CASES)
+@pytest.mark.parametrize("fwd", [True, False])
+def test_tile_radix16_schedules_bit_exact(lf, oracle, N, hm, rs, fwd):
+    run_case(lf, oracle, N, hm, fwd, rs)
+
+
+def run_case(lf, oracle, N, hm, fwd, rs):
     x = oracle.lcg_input(N, seed=0x7E1 ^ N ^ (fwd << 33), howmany=hm)
-    plan = lf.Plan(N, lf.FORWARD if fwd else lf.BACKWARD, hm)
+    plan = lf.Plan(N, lf.FORWARD if fwd else lf.BACKWARD, hm, radices=rs)
     ref = oracle.ref_transform(fwd, plan.radices, x)
     xd = torch.from_numpy(np.ascontiguousarray(x)).to(DEV)
     out = torch.empty_like(xd)
