@@ -72,7 +72,9 @@ struct StageArgs {
   int ept;      // elements per thread: 8 (G*L <= LDS_ELEMS, 256 threads) or 16 (large tiles,
                 // G*L <= BIG_ELEMS, up to 512 threads)
   int gfast_in, gfast_out;  // 1: consecutive threads walk g (g adjacent in memory)
-  int stage_io;             // 1: coalesced tile copies HBM -> LDS and back (contiguous small L)
+  int stage_io;             // 1: coalesced tile copies HBM -> LDS and back (contiguous small
+                            // L); 2: the same in the complex part of a real-data plan, whose
+                            // tile kernel takes the real-data maps into its copies
   int reg;                  // 1: register-resident kernel (mfft_reg.hip), one transform of
                             // L = 32..16384 points, L / 8 or L / 16 threads per transform;
                             // 2: register-resident stage kernel, G sub-problems of L / 8

@@ -63,13 +63,14 @@ const void *kernel_for(int fwd, int set, int ept, int sq, int big) {
 hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,
                         double2 *out, const double2 *tw, hipStream_t stream) {
   if (a.nsub <= 0 || ntrans <= 0) return hipSuccess;
-  // the real-data element maps exist in the register kernels only
-  // (input maps at 16 elements per thread: real input into a schedule opening with radix 16)
-  if ((a.io_in || a.io_out) &&
+  // the real-data element maps exist in the register kernels (input maps at 16 elements per
+  // thread: real input into a schedule opening with radix 16) and in the staged tile kernels
+  const bool tile_io = a.tile > 0 && a.stage_io == 2 && !a.pre && !a.post && !a.sumsq;
+  if ((a.io_in || a.io_out) && !tile_io &&
       (a.reg == 0 || (a.reg == 2 && a.sumsq) ||
        (a.io_in && a.ept != 8 && !(a.io_in == 1 && a.reg == 1 && a.pass[0].R == 16))))
     return hipErrorInvalidValue;
-  if (a.tile > 0 && !a.pre && !a.post && !a.sumsq && !a.io_in && !a.io_out)
+  if (a.tile > 0 && !a.pre && !a.post && !a.sumsq && (tile_io || (!a.io_in && !a.io_out)))
     return launch_tile(fwd, a, ntrans, in, out, tw, stream);
   if (a.reg == 1) return ntrans == 1 ? launch_reg(fwd, a, in, out, tw, stream) : hipErrorInvalidValue;
   if (a.reg == 2 && !a.sumsq) return launch_reg_stage(fwd, a, ntrans, in, out, tw, stream);

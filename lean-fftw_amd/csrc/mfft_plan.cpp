@@ -49,6 +49,7 @@ struct mfft_plan {
   int device = 0;
   int64_t spec_in_off = 0, spec_out_off = 0;  // doubles, for mfft_plan_spec plans
   bool allow_big = true;  // may use the large-tile single-stage variant
+  bool fuse = false;       // the complex part of a real-data plan (schedules the maps fuse into)
   // composite plans, built from complex plans (parts) the way the reference's generic engine
   // builds them (LeanFFTW/Core/Problem.lean:215-236): 0 complex DFT; 1 real to half complex
   // (r2c, part 0 the forward DFT on the embedded batch in d_work); 2 half complex to real (c2r,
@@ -429,8 +430,11 @@ bool build_stages(mfft_plan *p, int split) {
   std::vector<int64_t> ns(s + 1, 1);
   for (int t = 0; t < s; ++t) ns[t + 1] = ns[t] * rs[t];
   if (split <= 0 || split >= s) {
+    // the complex part of a real-data plan runs a staged tile kernel up to 64 points, whose
+    // tile copies take the real-data maps (the register kernel's fused maps from 128 on)
+    constexpr int64_t fuse_tile_max = 64;
     const bool reg = p->allow_big && reg_size(N) && p->istride == 1 && p->ostride == 1 &&
-                     mfft::reg_supported(N, rs.data(), s);
+                     mfft::reg_supported(N, rs.data(), s) && !(p->fuse && N <= fuse_tile_max);
     const bool big = !reg && N > mfft::LDS_ELEMS && p->allow_big &&
                      N <= ((N & (N - 1)) == 0 ? mfft::BIG_ELEMS : mfft::BIG_ELEMS / 2);
     if ((N > mfft::LDS_ELEMS && !big && !reg) || s > mfft::MAX_PASSES) return false;
@@ -469,9 +473,10 @@ bool build_stages(mfft_plan *p, int split) {
     a.post_e = 1;
     // small contiguous transforms: stage the tile through LDS with coalesced copies
     a.m_L = mfft::magic32(N);
-    a.stage_io = (N <= stage_io_max() && a.ept == 8 && p->istride == 1 && p->ostride == 1 &&
+    const int64_t io_max = p->fuse ? std::max<int64_t>(fuse_tile_max, stage_io_max()) : stage_io_max();
+    a.stage_io = (N <= io_max && a.ept == 8 && p->istride == 1 && p->ostride == 1 &&
                   p->idist == N && p->odist == N)
-                     ? 1
+                     ? (p->fuse ? 2 : 1)
                      : 0;
     // the staged tile is read with ordinary loads (MI355X, tile kernels, 256 MB batches of
     // 2..16 points: 87-90 us with non-temporal loads, 80-82 with ordinary ones)
@@ -657,6 +662,8 @@ int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
 // per thread, or real input into a 16-per-thread schedule that opens with radix 16).
 bool fusable(const mfft_plan *q, int io_in) {
   if (q->istride != 1 || q->ostride != 1 || q->columns || q->stages.empty()) return false;
+  // a single stage in a staged tile kernel with maps takes them into its tile copies
+  if (q->stages.size() == 1 && q->stages[0].tile > 0 && q->stages[0].stage_io == 2) return true;
   for (const auto &a : q->stages)
     if (a.reg == 0) return false;
   const StageArgs &a0 = q->stages[0];
@@ -928,6 +935,7 @@ mfft_plan *clone_shape(const mfft_plan *p, int64_t howmany) {
   q->ostride = p->ostride;
   q->odist = p->odist;
   q->allow_big = p->allow_big;
+  q->fuse = p->fuse;
   return q;
 }
 
@@ -1002,6 +1010,7 @@ mfft_plan *plan_many_internal(int64_t n, int sign, int64_t howmany, int64_t istr
   p->ostride = ostride;
   p->odist = odist;
   p->allow_big = allow_big;
+  p->fuse = (flags & kFuseRadices) != 0;
   const bool pow2 = (n & (n - 1)) == 0;
   int split = 0;
   if (flags & MFFT_MEASURE) {

@@ -15,7 +15,9 @@
 // with IO, a coalesced copy stages the tile through LDS first), the intermediate passes run in
 // place in one padded LDS buffer, the last pass writes HBM (or the tile is copied out).
 // GFI / GFO: consecutive lanes walk the sub-problem index on the input / output side (the
-// sub-problems are adjacent in memory), else the butterfly index.
+// sub-problems are adjacent in memory), else the butterfly index. IO: 0 the tile is read and
+// written by the first and last passes, 1 staged through LDS by coalesced tile copies, 2 staged
+// with the real-data maps of r2c / c2r plans in the tile copies.
 #include <cstdlib>
 #include <mutex>
 #include <set>
@@ -116,16 +118,44 @@ __device__ __forceinline__ void tile_passes(double2 *buf, const HbmSide &hin, co
 }
 
 // Coalesced copies of a tile of contiguous sub-problems (sub-problem g at g * L) between HBM
-// and LDS: consecutive lanes take consecutive elements.
-template <int L, int G, int NT>
-__device__ __forceinline__ void tile_in(double2 *buf, const HbmSide &h, int tid, int gcount) {
+// and LDS: consecutive lanes take consecutive elements. With the real-data maps of a staged
+// tile (contiguous transforms of length L, rows of in_g / out_g elements of the arrays' own
+// types, `ib` / `ob` the tile's first row): io_in 1 reads doubles as (x, 0), 2 the Hermitian
+// extension of L/2 + 1 complex values; io_out 1 stores real parts, 2 the first L/2 + 1 values.
+template <int L, int G, int NT, bool MAPS>
+__device__ __forceinline__ void tile_in(double2 *buf, const HbmSide &h, int tid, int gcount,
+                                        const StageArgs &a, const void *ib) {
   constexpr int EP = (G * L + NT - 1) / NT, LL = lds_len(L);
   const int total = gcount * L;
   double2 v[EP];
+  if (!MAPS || a.io_in == 0) {
 #pragma unroll
-  for (int u = 0; u < EP; ++u) {
-    const int f = tid + u * NT;
-    if (f < total) v[u] = h.cached ? h.base[f] : ld_stream(h.base + f);
+    for (int u = 0; u < EP; ++u) {
+      const int f = tid + u * NT;
+      if (f < total) v[u] = h.cached ? h.base[f] : ld_stream(h.base + f);
+    }
+  } else if (a.io_in == 1) {
+    const double *src = static_cast<const double *>(ib);
+#pragma unroll
+    for (int u = 0; u < EP; ++u) {
+      const int f = tid + u * NT;
+      if (f < total) {
+        const int g = f / L;
+        v[u] = make_double2(src[g * a.in_g + (f - g * L)], 0.0);
+      }
+    }
+  } else {
+    const double2 *src = static_cast<const double2 *>(ib);
+#pragma unroll
+    for (int u = 0; u < EP; ++u) {
+      const int f = tid + u * NT;
+      if (f < total) {
+        const int g = f / L, e = f - g * L;
+        const bool mirror = e > L / 2;
+        v[u] = src[g * a.in_g + (mirror ? L - e : e)];
+        if (mirror) v[u].y = -v[u].y;
+      }
+    }
   }
 #pragma unroll
   for (int u = 0; u < EP; ++u) {
@@ -138,21 +168,27 @@ __device__ __forceinline__ void tile_in(double2 *buf, const HbmSide &h, int tid,
   __syncthreads();
 }
 
-template <int L, int G, int NT>
+template <int L, int G, int NT, bool MAPS>
 __device__ __forceinline__ void tile_out(const double2 *buf, const HbmSide &h, int tid,
-                                         int gcount) {
+                                         int gcount, const StageArgs &a, void *ob) {
   constexpr int EP = (G * L + NT - 1) / NT, LL = lds_len(L);
   const int total = gcount * L;
 #pragma unroll
   for (int u = 0; u < EP; ++u) {
     const int f = tid + u * NT;
     if (f < total) {
-      const int g = f / L;
-      const double2 v = buf[g * LL + lpad(f - g * L)];
-      if (h.cached)
-        h.base[f] = v;
-      else
-        st_stream(h.base + f, v);
+      const int g = f / L, e = f - g * L;
+      const double2 v = buf[g * LL + lpad(e)];
+      if (!MAPS || a.io_out == 0) {
+        if (h.cached)
+          h.base[f] = v;
+        else
+          st_stream(h.base + f, v);
+      } else if (a.io_out == 1) {
+        __builtin_nontemporal_store(v.x, static_cast<double *>(ob) + g * a.out_g + e);
+      } else if (e <= L / 2) {
+        st_stream(static_cast<double2 *>(ob) + g * a.out_g + e, v);
+      }
     }
   }
 }
@@ -172,10 +208,20 @@ __global__ __launch_bounds__(NT) void tile_kernel(const double2 *in, double2 *ou
   const HbmSide hout{out + h * a.out_h + g0 * a.out_g, nullptr, a.out_g, a.out_e, nullptr, 0, 0,
                      a.cached_out};
   const int tid = threadIdx.x;
-  if constexpr (IO != 0) tile_in<L, G, NT>(lds, hin, tid, gcount);
+  constexpr bool MAPS = IO == 2;
+  if constexpr (IO != 0) {
+    // the real-data maps address the arrays in their own element types
+    const size_t isz = MAPS && a.io_in == 1 ? sizeof(double) : sizeof(double2);
+    const void *ib = reinterpret_cast<const char *>(in) + (h * a.in_h + (int64_t)g0 * a.in_g) * isz;
+    tile_in<L, G, NT, MAPS>(lds, hin, tid, gcount, a, ib);
+  }
   tile_passes<FWD, L, G, NT, GFI, GFO, IO, 0, 1, Rs...>(lds, hin, hout, tid, gcount, a,
                                                         tw + h * a.kb_h, g0);
-  if constexpr (IO != 0) tile_out<L, G, NT>(lds, hout, tid, gcount);
+  if constexpr (IO != 0) {
+    const size_t osz = MAPS && a.io_out == 1 ? sizeof(double) : sizeof(double2);
+    void *ob = reinterpret_cast<char *>(out) + (h * a.out_h + (int64_t)g0 * a.out_g) * osz;
+    tile_out<L, G, NT, MAPS>(lds, hout, tid, gcount, a, ob);
+  }
 }
 
 // The fixed stage shapes: sub-problem length, radix sequence, lane order per side, tile copies,
@@ -206,6 +252,17 @@ const TileSchedule kTiles[] = {
     MFFT_T(8, 1, RS_(8), 0, 0, 1, 256, 256, 8),
     MFFT_T(16, 1, RS_(16), 0, 0, 1, 128, 256, 16),
     MFFT_T(6, 1, RS_(6), 0, 0, 1, 256, 256, 6),
+    // the complex parts of real-data plans up to 64 points: staged tiles whose copies take
+    // the real-data maps (tile copies 2; r2c per 256 MB of complex-equivalent batch on MI355X:
+    // 16 and 32 points 221 / 207 -> 44 / 52 us against the unfused composition, 64 points
+    // 65 -> 52 against the register kernel's fused maps, which are faster from 128 points)
+    MFFT_T(2, 1, RS_(2), 0, 0, 2, 1024, 256, 2),
+    MFFT_T(4, 1, RS_(4), 0, 0, 2, 512, 256, 4),
+    MFFT_T(8, 1, RS_(8), 0, 0, 2, 256, 256, 8),
+    MFFT_T(16, 1, RS_(16), 0, 0, 2, 128, 256, 16),
+    MFFT_T(6, 1, RS_(6), 0, 0, 2, 256, 256, 6),
+    MFFT_T(32, 2, RS_(8, 4), 0, 0, 2, 64, 256, 8, 4),
+    MFFT_T(64, 2, RS_(8, 8), 0, 0, 2, 32, 256, 8, 8),
     // single-stage contiguous batches read butterfly by butterfly; from 64 points on in place
     // of the register-resident kernel (MI355X, 256 MB batches: 64 ... 1024 points 78-83 us
     // against 88-98, 2048 points 82 against 90; radix 16 measured 1-2 us slower at 256, 512
@@ -267,7 +324,7 @@ int tile_lookup(const StageArgs &a) {
   for (int i = 0; i < kNumTiles; ++i) {
     const TileSchedule &t = kTiles[i];
     if (t.L != a.L || t.npass != a.npass || t.gfi != (a.gfast_in ? 1 : 0) ||
-        t.gfo != (a.gfast_out ? 1 : 0) || t.io != (a.stage_io ? 1 : 0) || a.nsub < t.G)
+        t.gfo != (a.gfast_out ? 1 : 0) || t.io != a.stage_io || a.nsub < t.G)
       continue;
     bool same = true;
     for (int p = 0; p < a.npass; ++p) same = same && t.rs[p] == a.pass[p].R;

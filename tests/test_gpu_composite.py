@@ -52,8 +52,10 @@ def herm_ext(h, n):
                     np.conj(h[:, np.minimum((n - k) % n, n // 2)]))
 
 
-# powers of two from 64 on run with the maps fused into the register kernels (16384 through
-# the embedded batch on the input side); the others through the element-map kernels
+# powers of two from 128 on run with the maps fused into the register kernels (16384 through
+# the embedded batch on the input side), batches of small sizes that fill a staged tile with
+# the maps in the tile copies (tests/test_gpu_tile.py); the others through the element-map
+# kernels
 REAL_SIZES = [2, 3, 4, 5, 8, 15, 16, 32, 60, 64, 105, 128, 256, 512, 1000, 1024, 2048, 2100,
               4096, 8192, 10368, 16384, 32768, 65536, 131072, 165375, 262144]
 

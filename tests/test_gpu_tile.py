@@ -84,3 +84,33 @@ def test_column_tile_kernels_bit_exact(lf, oracle, N, hm, fwd):
     torch.cuda.synchronize()
     assert bits_equal(out.cpu().numpy(), ref), (N, plan.radices, plan.stages, "out of place")
     assert bits_equal(xt.cpu().numpy(), ref), (N, plan.radices, plan.stages, "in place")
+
+
+# real-data plans whose complex part is a staged tile kernel take the maps into its tile copies
+# (a batch of at least one tile: 2: 1024, 4: 512, 8: 256, 16: 128, 6: 256, 32: 64, 64: 32
+# transforms)
+REAL_TILE_CASES = [(2, 1100), (4, 600), (8, 300), (16, 200), (6, 300), (32, 70), (64, 40)]
+
+
+def _herm_ext(h, n):
+    """hermExt (Problem.lean:138-140): h_k for k <= n/2, conj(h_{n-k}) above."""
+    k = np.arange(n)
+    return np.where(k <= n // 2, h[:, np.minimum(k, n // 2)],
+                    np.conj(h[:, np.minimum((n - k) % n, n // 2)]))
+
+
+@pytest.mark.parametrize("n,hm", REAL_TILE_CASES)
+def test_real_data_maps_in_tile_copies_bit_exact(lf, oracle, n, hm):
+    rng = np.random.default_rng(n + hm)
+    x = rng.standard_normal((hm, n))
+    plan = lf.RealPlan(n, lf.FORWARD, hm)
+    got = plan.exec(torch.from_numpy(x).to(DEV))
+    torch.cuda.synchronize()
+    full = oracle.ref_transform(True, plan.radices, x.astype(np.complex128))
+    assert bits_equal(got.cpu().numpy(), full[:, : n // 2 + 1]), (n, "r2c")
+    h = rng.standard_normal((hm, n // 2 + 1)) + 1j * rng.standard_normal((hm, n // 2 + 1))
+    plan = lf.RealPlan(n, lf.BACKWARD, hm)
+    got = plan.exec(torch.from_numpy(h).to(DEV))
+    torch.cuda.synchronize()
+    full = oracle.ref_transform(False, plan.radices, _herm_ext(h, n))
+    assert bits_equal(got.cpu().numpy(), np.ascontiguousarray(full.real)), (n, "c2r")
