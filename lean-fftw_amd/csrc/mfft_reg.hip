@@ -549,6 +549,7 @@ const StageSchedule kStages[] = {
     {512, 16, 3, {8, 8, 8}, false, MFFT_SK(512, 16, 8, 8, 8)},
     {1024, 4, 4, {8, 8, 8, 2}, false, MFFT_SK(1024, 4, 8, 8, 8, 2)},
     {1024, 8, 4, {8, 8, 8, 2}, true, MFFT_SK(1024, 8, 8, 8, 8, 2)},
+    {2048, 4, 4, {8, 8, 8, 4}, true, MFFT_SK(2048, 4, 8, 8, 8, 4)},
 };
 #undef MFFT_SK
 

@@ -22,11 +22,14 @@ DEV = "cuda:0"
 
 # (N, batch): single-stage sizes with tiles of G transforms (2: 1024, 4: 512, 8: 256,
 # 16: 128, 6: 341, 32: 64, 60: 34, 360: 5, 1000: 2, 2100: 1) and two-stage sizes whose
-# stages have tile schedules (10368 = 96 x 108, 75600 = 270 x 280, 165375 = 375 x 441); 64 ...
+# stages have tile schedules (10368 = 96 x 108, 75600 = 270 x 280, 165375 = 375 x 441; their
+# second stages run XCD-remapped grids padded to a multiple of 8 tiles, over one or several
+# transforms); 64 ...
 # 2048 points run the tile kernels in place of the register-resident kernel, 4096 and the
 # two-stage power-of-two sizes keep the register kernels
 TILE_CASES = [(2, 1537), (4, 769), (8, 385), (16, 193), (6, 513), (32, 97), (60, 52),
               (360, 8), (1000, 3), (2100, 3), (10368, 2), (75600, 1), (165375, 1),
+              (75600, 3), (165375, 2),
               (64, 49), (128, 25), (256, 13), (512, 7), (1024, 3), (2048, 2), (4096, 2),
               (32768, 1), (65536, 1), (131072, 1), (262144, 1)]
 
