@@ -121,6 +121,20 @@ def test_generated_header_is_current():
     assert open(path).read() == cg.emit_all()
 
 
+def test_generated_tile_tables_are_current(lf):
+    """The committed tables of csrc/mfft_tile_gen*.hip are what lean-fftw_amd/tilegen.py writes
+    for the planner as built: every shape's radices are the size's default schedule."""
+    spec = importlib.util.spec_from_file_location("mfft_tilegen",
+                                                  os.path.join(ROOT, "lean-fftw_amd", "tilegen.py"))
+    tg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(tg)
+    ents = tg.entries(lf.lib())
+    parts = [ents[i::tg.PARTS] for i in range(tg.PARTS)]
+    for i, pe in enumerate(parts):
+        path = os.path.join(ROOT, "lean-fftw_amd", "csrc", f"mfft_tile_gen{i}.hip")
+        assert open(path).read() == tg.render(i, pe), path
+
+
 def test_missing_library_fails_loudly(lf, tmp_path, monkeypatch):
     monkeypatch.setattr(lf, "LIB_PATH", str(tmp_path / "nope.so"))
     monkeypatch.setattr(lf, "_lib", None)

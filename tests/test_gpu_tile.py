@@ -117,3 +117,60 @@ def test_real_data_maps_in_tile_copies_bit_exact(lf, oracle, n, hm):
     torch.cuda.synchronize()
     full = oracle.ref_transform(False, plan.radices, _herm_ext(h, n))
     assert bits_equal(got.cpu().numpy(), np.ascontiguousarray(full.real)), (n, "c2r")
+
+
+# the generated tables (csrc/mfft_tile_gen*.hip): every 13-smooth size up to 4096 points that is
+# not a power of two, through its default plan, with one full tile and a partial one
+def _generated_shapes():
+    import os
+    import re
+
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "lean-fftw_amd", "csrc")
+    out = []
+    for i in range(8):
+        with open(os.path.join(here, f"mfft_tile_gen{i}.hip")) as f:
+            for m in re.finditer(r"MFFT_T\((\d+), \d+, RS_\(([\d, ]+)\), 0, 0, (\d), (\d+), (\d+),",
+                                 f.read()):
+                out.append((int(m.group(1)), [int(x) for x in m.group(2).split(",")],
+                            int(m.group(3)), int(m.group(4))))
+    return sorted(out)
+
+
+GEN = _generated_shapes()
+GEN_COMPLEX = [(n, rs, G) for n, rs, io, G in GEN if io != 2]
+GEN_REAL = [(n, rs, G) for n, rs, io, G in GEN if io == 2]
+
+
+@pytest.mark.parametrize("n,rs,G", GEN_COMPLEX, ids=[str(g[0]) for g in GEN_COMPLEX])
+def test_generated_tile_kernels_bit_exact(lf, oracle, n, rs, G):
+    hm = G + G // 2 + 1
+    fwd = n % 2 == 0  # both directions over the set
+    x = oracle.lcg_input(n, seed=0x6E7 ^ n, howmany=hm)
+    plan = lf.Plan(n, lf.FORWARD if fwd else lf.BACKWARD, hm)
+    assert plan.radices == rs, (n, plan.radices, rs)
+    ref = oracle.ref_transform(fwd, rs, x)
+    xd = torch.from_numpy(np.ascontiguousarray(x)).to(DEV)
+    out = torch.empty_like(xd)
+    plan.call(xd, out)
+    torch.cuda.synchronize()
+    assert bits_equal(out.cpu().numpy(), ref), (n, rs)
+
+
+@pytest.mark.parametrize("n,rs,G", GEN_REAL, ids=[str(g[0]) for g in GEN_REAL])
+def test_generated_real_data_tiles_bit_exact(lf, oracle, n, rs, G):
+    hm = G + G // 2 + 1
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((hm, n))
+    plan = lf.RealPlan(n, lf.FORWARD, hm)
+    assert plan.radices == rs, (n, plan.radices, rs)
+    got = plan.exec(torch.from_numpy(x).to(DEV))
+    torch.cuda.synchronize()
+    full = oracle.ref_transform(True, rs, x.astype(np.complex128))
+    assert bits_equal(got.cpu().numpy(), full[:, : n // 2 + 1]), (n, "r2c")
+    h = rng.standard_normal((hm, n // 2 + 1)) + 1j * rng.standard_normal((hm, n // 2 + 1))
+    plan = lf.RealPlan(n, lf.BACKWARD, hm)
+    got = plan.exec(torch.from_numpy(h).to(DEV))
+    torch.cuda.synchronize()
+    full = oracle.ref_transform(False, rs, _herm_ext(h, n))
+    assert bits_equal(got.cpu().numpy(), np.ascontiguousarray(full.real)), (n, "c2r")
