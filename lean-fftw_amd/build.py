@@ -31,6 +31,7 @@ SOURCES = [
     ("mfft_kernels.hip", True),
     ("mfft_reg.hip", True),
     ("mfft_tile.hip", True),
+    ("mfft_tile2d.hip", True),
     *((f"mfft_tile_gen{i}.hip", True) for i in range(8)),
     ("mfft_real.hip", True),
     ("mfft_ragged.hip", True),

@@ -154,6 +154,14 @@ int xcd_remap(const StageArgs &a);
 hipError_t launch_tile(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,
                        double2 *out, const double2 *tw, hipStream_t stream);
 
+// Single-pass 2-D transforms of small arrays (mfft_tile2d.hip): the fused kernel for an
+// n0 x n1 plan whose axis-0 (col) and axis-1 (row) parts are single-stage with these stages'
+// radices (1 + its index, 0 if none), and its launch over `howmany` contiguous arrays.
+int tile2d_lookup(int64_t n0, int64_t n1, const StageArgs &col, const StageArgs &row);
+hipError_t launch_tile2d(bool fwd, int idx, const StageArgs &col, const StageArgs &row,
+                         const double2 *in, double2 *out, const double2 *tw0,
+                         const double2 *tw1, int64_t howmany, hipStream_t stream);
+
 // Launch one stage over `ntrans` transforms; returns a hipError_t.
 hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,
                         double2 *out, const double2 *tw, hipStream_t stream);

@@ -24,7 +24,18 @@
 
 namespace mfft {
 
-template <bool FWD, int R, int L, int G, int NT, int NSL, bool SRC_HBM, bool DST_HBM, bool GF>
+// LDS position of element e of sub-problem g: sub-problems contiguous and padded (CST = 0), or,
+// for the columns of a 2-D array held row-major in LDS, row e at e * CST and column g at lpad(g).
+template <int LL, int CST>
+__device__ __forceinline__ int lds_pos(int g, int e) {
+  if constexpr (CST == 0)
+    return g * LL + lpad(e);
+  else
+    return e * CST + lpad(g);
+}
+
+template <bool FWD, int R, int L, int G, int NT, int NSL, bool SRC_HBM, bool DST_HBM, bool GF,
+          int CST = 0>
 __device__ __forceinline__ void tile_pass(double2 *buf, const HbmSide &hin, const HbmSide &hout,
                                           int tid, int gcount, const double2 *__restrict__ tw,
                                           const PassArgs &pa, int64_t kb0, int64_t kb_u) {
@@ -61,9 +72,8 @@ __device__ __forceinline__ void tile_pass(double2 *buf, const HbmSide &hin, cons
           for (int r = 0; r < R; ++r) x[u][r] = ld_stream(s + r * st);
         }
       } else {
-        const double2 *s = buf + g * LL;
 #pragma unroll
-        for (int r = 0; r < R; ++r) x[u][r] = s[lpad(j + r * LR)];
+        for (int r = 0; r < R; ++r) x[u][r] = buf[lds_pos<LL, CST>(g, j + r * LR)];
       }
     }
   }
@@ -92,9 +102,8 @@ __device__ __forceinline__ void tile_pass(double2 *buf, const HbmSide &hin, cons
         for (int r = 0; r < R; ++r) st_stream(d + r * st, x[u][r]);
       }
     } else {
-      double2 *d = buf + g * LL;
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[lpad(e0 + r * NSL)] = x[u][r];
+      for (int r = 0; r < R; ++r) buf[lds_pos<LL, CST>(g, e0 + r * NSL)] = x[u][r];
     }
   }
   if constexpr (!DST_HBM) __syncthreads();
