@@ -86,7 +86,7 @@ struct Tile2D {
 using R88 = RList<8, 8>;
 using R84 = RList<8, 4>;
 const Tile2D k2D[] = {
-    MFFT_T2(64, 64, 2, RS_(8, 8), 2, RS_(8, 8), 256, R88, R88),
+    MFFT_T2(64, 64, 2, RS_(8, 8), 2, RS_(8, 8), 512, R88, R88),
     MFFT_T2(32, 32, 2, RS_(8, 4), 2, RS_(8, 4), 256, R84, R84),
     MFFT_T2(32, 64, 2, RS_(8, 4), 2, RS_(8, 8), 256, R84, R88),
     MFFT_T2(64, 32, 2, RS_(8, 8), 2, RS_(8, 4), 256, R88, R84),
