@@ -83,13 +83,27 @@ struct Tile2D {
    {(const void *)tile2d_kernel<false, A, B, NN, C, W>,                                      \
     (const void *)tile2d_kernel<true, A, B, NN, C, W>}}
 #define RS_(...) {__VA_ARGS__}
-using R88 = RList<8, 8>;
+// the default radices of the parts: 16 = 16, 32 = 8 x 4, 64 = 8 x 8, 128 = 8 x 8 x 2 (columns
+// and rows alike); 512 threads for 4096 points (64 x 64: 93 us per 2^24 elements against 105 at
+// 256), 256 below
+using R16 = RList<16>;
 using R84 = RList<8, 4>;
+using R88 = RList<8, 8>;
+using R882 = RList<8, 8, 2>;
 const Tile2D k2D[] = {
     MFFT_T2(64, 64, 2, RS_(8, 8), 2, RS_(8, 8), 512, R88, R88),
+    MFFT_T2(32, 128, 2, RS_(8, 4), 3, RS_(8, 8, 2), 512, R84, R882),
+    MFFT_T2(128, 32, 3, RS_(8, 8, 2), 2, RS_(8, 4), 512, R882, R84),
     MFFT_T2(32, 32, 2, RS_(8, 4), 2, RS_(8, 4), 256, R84, R84),
     MFFT_T2(32, 64, 2, RS_(8, 4), 2, RS_(8, 8), 256, R84, R88),
     MFFT_T2(64, 32, 2, RS_(8, 8), 2, RS_(8, 4), 256, R88, R84),
+    MFFT_T2(16, 16, 1, RS_(16), 1, RS_(16), 256, R16, R16),
+    MFFT_T2(16, 32, 1, RS_(16), 2, RS_(8, 4), 256, R16, R84),
+    MFFT_T2(32, 16, 2, RS_(8, 4), 1, RS_(16), 256, R84, R16),
+    MFFT_T2(16, 64, 1, RS_(16), 2, RS_(8, 8), 256, R16, R88),
+    MFFT_T2(64, 16, 2, RS_(8, 8), 1, RS_(16), 256, R88, R16),
+    MFFT_T2(16, 128, 1, RS_(16), 3, RS_(8, 8, 2), 256, R16, R882),
+    MFFT_T2(128, 16, 3, RS_(8, 8, 2), 1, RS_(16), 256, R882, R16),
 };
 #undef RS_
 #undef MFFT_T2

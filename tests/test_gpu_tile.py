@@ -189,7 +189,10 @@ def _oracle_2d(lf, oracle, fwd, x):
     return oracle.ref_transform(fwd, rs1, rows).reshape(hm, n0, n1)
 
 
-@pytest.mark.parametrize("n0,n1,hm", [(64, 64, 3), (32, 32, 5), (32, 64, 2), (64, 32, 2)])
+@pytest.mark.parametrize("n0,n1,hm", [(64, 64, 3), (32, 32, 5), (32, 64, 2), (64, 32, 2),
+                                      (32, 128, 2), (128, 32, 2), (16, 16, 4), (16, 32, 3),
+                                      (32, 16, 3), (16, 64, 2), (64, 16, 2), (16, 128, 2),
+                                      (128, 16, 2)])
 @pytest.mark.parametrize("fwd", [True, False])
 def test_fused_2d_small_arrays_bit_exact(lf, oracle, n0, n1, hm, fwd):
     rng = np.random.default_rng(n0 * 1000 + n1 + hm)
