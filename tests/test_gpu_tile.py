@@ -46,7 +46,8 @@ def test_tile_kernels_bit_exact(lf, oracle, N, hm, fwd):
 
 
 # the radix-16 tile schedules with explicit radices: a tile of two and a single transform
-RADIX16_CASES = [(1024, 3, [16, 8, 8]), (1024, 1, [16, 8, 8])]
+RADIX16_CASES = [(1024, 3, [16, 8, 8]), (1024, 1, [16, 8, 8]), (2048, 2, [16, 8, 16]),
+                 (2048, 1, [16, 8, 16])]
 
 
 @pytest.mark.parametrize("N,hm,rs", RADIX16_CASES)
