@@ -57,7 +57,15 @@ const TileSchedule kTiles[] = {
     MFFT_T(1000, 3, RS_(10, 10, 10), 0, 0, 0, 2, 256, 10, 10, 10),
     MFFT_T(2100, 3, RS_(15, 14, 10), 0, 0, 0, 1, 256, 15, 14, 10),
     // stages of two-stage plans: A reads rows of G adjacent columns and writes each column
-    // contiguously; B reads and writes rows of G adjacent residue classes
+    // contiguously; B reads and writes rows of G adjacent residue classes. The wider tiles come
+    // first (MI355X, 256 MB batches: 10368 = 96 x 108 at G = 42 / 36 on 512 threads 211 ->
+    // 183 us against G = 21 / 18 on 256; 165375's 441-point stage at G = 16 on 1024 threads
+    // 315 -> 285 us against G = 8 on 512, 295 at G = 16 on 512; wider tiles of 270, 280 and
+    // 375 points measured equal or slower), the narrower ones take stages of fewer
+    // sub-problems
+    MFFT_T(96, 2, RS_(12, 8), 1, 0, 0, 42, 512, 12, 8),
+    MFFT_T(108, 2, RS_(12, 9), 1, 1, 0, 36, 512, 12, 9),
+    MFFT_T(441, 3, RS_(9, 7, 7), 1, 1, 0, 16, 1024, 9, 7, 7),
     MFFT_T(96, 2, RS_(12, 8), 1, 0, 0, 21, 256, 12, 8),
     MFFT_T(108, 2, RS_(12, 9), 1, 1, 0, 18, 256, 12, 9),
     MFFT_T(270, 3, RS_(10, 9, 3), 1, 0, 0, 8, 256, 10, 9, 3),

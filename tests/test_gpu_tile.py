@@ -56,9 +56,24 @@ def test_tile_radix16_schedules_bit_exact(lf, oracle, N, hm, rs, fwd):
     run_case(lf, oracle, N, hm, fwd, rs)
 
 
-def run_case(lf, oracle, N, hm, fwd, rs):
+# two-stage stage shapes with two tiles in the table, the wider first: stages with fewer
+# sub-problems than the wide tile holds take the narrow one (2304 = 96 x 24: 24 columns of 96
+# points, tiles of 21 rather than 42; 2592 = 24 x 108: 24 residue classes of 108, tiles of 18
+# rather than 36; 4410 = 10 x 441: 10 residue classes of 441, tiles of 8 rather than 16)
+NARROW_STAGE_CASES = [(2304, [12, 8, 12, 2], 2), (2592, [12, 2, 12, 9], 2),
+                      (4410, [10, 9, 7, 7], 1)]
+
+
+@pytest.mark.parametrize("N,rs,split", NARROW_STAGE_CASES)
+@pytest.mark.parametrize("fwd", [True, False])
+def test_narrow_stage_tiles_bit_exact(lf, oracle, N, rs, split, fwd):
+    run_case(lf, oracle, N, 3, fwd, rs, split)
+
+
+def run_case(lf, oracle, N, hm, fwd, rs, split=None):
     x = oracle.lcg_input(N, seed=0x7E1 ^ N ^ (fwd << 33), howmany=hm)
-    plan = lf.Plan(N, lf.FORWARD if fwd else lf.BACKWARD, hm, radices=rs)
+    kw = {} if split is None else {"split": split}
+    plan = lf.Plan(N, lf.FORWARD if fwd else lf.BACKWARD, hm, radices=rs, **kw)
     ref = oracle.ref_transform(fwd, plan.radices, x)
     xd = torch.from_numpy(np.ascontiguousarray(x)).to(DEV)
     out = torch.empty_like(xd)
