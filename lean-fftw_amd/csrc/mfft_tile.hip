@@ -60,12 +60,14 @@ const TileSchedule kTiles[] = {
     // contiguously; B reads and writes rows of G adjacent residue classes. The wider tiles come
     // first (MI355X, 256 MB batches: 10368 = 96 x 108 at G = 42 / 36 on 512 threads 211 ->
     // 183 us against G = 21 / 18 on 256; 165375's 441-point stage at G = 16 on 1024 threads
-    // 315 -> 285 us against G = 8 on 512, 295 at G = 16 on 512; wider tiles of 270, 280 and
-    // 375 points measured equal or slower), the narrower ones take stages of fewer
-    // sub-problems
+    // 315 -> 285 us against G = 8 on 512, 295 at G = 16 on 512; 75600's 280-point stage at
+    // G = 12 on 512 threads 218-222 -> 212-213 us against G = 8 on 256, G = 8 or 16 and 384 or
+    // 768 threads slower; wider tiles of 270 and 375 points measured equal or slower), the
+    // narrower ones take stages of fewer sub-problems
     MFFT_T(96, 2, RS_(12, 8), 1, 0, 0, 42, 512, 12, 8),
     MFFT_T(108, 2, RS_(12, 9), 1, 1, 0, 36, 512, 12, 9),
     MFFT_T(441, 3, RS_(9, 7, 7), 1, 1, 0, 16, 1024, 9, 7, 7),
+    MFFT_T(280, 3, RS_(10, 7, 4), 1, 1, 0, 12, 512, 10, 7, 4),
     MFFT_T(96, 2, RS_(12, 8), 1, 0, 0, 21, 256, 12, 8),
     MFFT_T(108, 2, RS_(12, 9), 1, 1, 0, 18, 256, 12, 9),
     MFFT_T(270, 3, RS_(10, 9, 3), 1, 0, 0, 8, 256, 10, 9, 3),

@@ -59,9 +59,10 @@ def test_tile_radix16_schedules_bit_exact(lf, oracle, N, hm, rs, fwd):
 # two-stage stage shapes with two tiles in the table, the wider first: stages with fewer
 # sub-problems than the wide tile holds take the narrow one (2304 = 96 x 24: 24 columns of 96
 # points, tiles of 21 rather than 42; 2592 = 24 x 108: 24 residue classes of 108, tiles of 18
-# rather than 36; 4410 = 10 x 441: 10 residue classes of 441, tiles of 8 rather than 16)
+# rather than 36; 4410 = 10 x 441: 10 residue classes of 441, tiles of 8 rather than 16;
+# 2800 = 10 x 280: 10 residue classes of 280, tiles of 8 rather than 12)
 NARROW_STAGE_CASES = [(2304, [12, 8, 12, 2], 2), (2592, [12, 2, 12, 9], 2),
-                      (4410, [10, 9, 7, 7], 1)]
+                      (4410, [10, 9, 7, 7], 1), (2800, [10, 10, 7, 4], 1)]
 
 
 @pytest.mark.parametrize("N,rs,split", NARROW_STAGE_CASES)
