@@ -1037,10 +1037,13 @@ mfft_plan *plan_many_internal(int64_t n, int sign, int64_t howmany, int64_t istr
   }
   const bool contig = istride == 1 && ostride == 1;
   // columns too long for a tile of 4 (64-byte rows) run as two stages of wider rows rather
-  // than one stage of narrower rows (MI355X, 2-D columns: 2048 points one stage of 4-wide
-  // rows 311 us against 340 for two stages, and 325 against 332 with the tile kernels; 4096
-  // points two stages 274-304 us against 868)
-  if (single_stage(n, allow_big, contig) && !(columns_layout(p) && 4 * n > mfft::BIG_ELEMS)) {
+  // than one stage of narrower rows (MI355X, 2-D columns: 4096 points two stages 274-304 us
+  // against 868), and so do power-of-two columns too long for a tile of 8, whose stages have
+  // column tiles (2048 x 2048: 285-287 us as 32 x 64 against 316-322 in one stage of 4-wide
+  // rows)
+  const bool col_two =
+      columns_layout(p) && (4 * n > mfft::BIG_ELEMS || (pow2 && 8 * n > mfft::BIG_ELEMS));
+  if (single_stage(n, allow_big, contig) && !col_two) {
     p->radices = allow_big && contig && reg_size(n) ? reg_radices(n, (flags & kFuseRadices) != 0)
                                                     : choose_radices(n, !allow_big);
   } else {

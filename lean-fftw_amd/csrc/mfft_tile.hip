@@ -77,8 +77,11 @@ const TileSchedule kTiles[] = {
     // batches of adjacent transforms (the columns of a row-major array: lanes walk the
     // transforms on both sides), and the stages of two-stage column plans (2-D 64 x 64:
     // 173 -> 167 us, 4096 x 4096: 276 -> 267 per 2^24 elements; a tile for the 2048-point
-    // columns, one workgroup per CU either way, measured the same as the stage kernel)
+    // columns, one workgroup per CU either way, measured the same as the stage kernel; the
+    // 32-point stage of 2048-point columns as 32 x 64: 2048 x 2048 316-322 -> 285-287 us, with
+    // tiles of 64 columns 325, of 16 or 32 on 128 threads 291-293, of 64 on 512 threads 323)
     MFFT_T(64, 2, RS_(8, 8), 1, 1, 0, 32, 256, 8, 8),
+    MFFT_T(32, 2, RS_(8, 4), 1, 1, 0, 32, 256, 8, 4),
 };
 #undef RS_
 #undef MFFT_T

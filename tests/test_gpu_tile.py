@@ -86,8 +86,9 @@ def run_case(lf, oracle, N, hm, fwd, rs, split=None):
 
 
 # columns of a row-major (N, hm) array: single-stage tiles of 32 adjacent transforms of 64
-# points, and both stages (64 x 64) of a two-stage column plan of 4096 points
-COLUMN_TILE_CASES = [(64, 40), (4096, 40)]
+# points, and both stages of the two-stage column plans of 4096 (64 x 64) and 2048 (32 x 64)
+# points
+COLUMN_TILE_CASES = [(64, 40), (4096, 40), (2048, 40)]
 
 
 @pytest.mark.parametrize("N,hm", COLUMN_TILE_CASES)
@@ -97,6 +98,7 @@ def test_column_tile_kernels_bit_exact(lf, oracle, N, hm, fwd):
     xt = torch.from_numpy(np.ascontiguousarray(x.T)).to(DEV)
     plan = lf.Plan(N, lf.FORWARD if fwd else lf.BACKWARD, hm, istride=hm, idist=1, ostride=hm,
                    odist=1)
+    assert N == 64 or plan.stages == [2, 2], (N, plan.radices, plan.stages)
     ref = np.ascontiguousarray(oracle.ref_transform(fwd, plan.radices, x).T)
     out = torch.empty_like(xt)
     plan.call(xt, out)
