@@ -87,16 +87,20 @@ const TileSchedule kTiles[] = {
 #undef MFFT_T
 constexpr int kNumTiles = (int)(sizeof(kTiles) / sizeof(kTiles[0]));
 
-// Whether a launch of this stage remaps its workgroups by XCD: stages whose twiddle base
-// depends on the sub-problem (kb_u != 0, the second stage of a two-stage plan, whose twiddles
-// span the whole table) and whose rows span at least 8 tiles. MI355X: 75600 points 233 ->
-// 218 us, 165375 points 331 -> 313 per 256 MB batch (MFFT_XCD=0 turns it off).
+// Whether a launch of this stage remaps its workgroups by XCD, for rows that span at least 8
+// tiles: stages whose twiddle base depends on the sub-problem (kb_u != 0, the second stage of
+// a two-stage plan, whose twiddles span the whole table; MI355X: 75600 points 233 -> 218 us,
+// 165375 points 331 -> 313 per 256 MB batch), and stages reading rows of adjacent
+// sub-problems whose stride is not a whole number of 128-byte lines, so that the tiles that
+// share a line meet in one L2 (165375's first stage, rows 7056 B apart: 288 -> 277-279 us;
+// 75600's, 4480 B apart, lost 11 us to it). MFFT_XCD=0 turns it off.
 int xcd_remap(const StageArgs &a) {
   static const int on = [] {
     const char *e = std::getenv("MFFT_XCD");
     return e ? std::atoi(e) : 1;
   }();
-  return on && a.kb_u != 0 && a.nsub >= 8 * (int64_t)a.G ? 1 : 0;
+  const bool split_lines = a.gfast_in && a.in_e % 8 != 0;
+  return on && (a.kb_u != 0 || split_lines) && a.nsub >= 8 * (int64_t)a.G ? 1 : 0;
 }
 
 extern const TileSchedule *const kTileGenPart0, *const kTileGenPart1, *const kTileGenPart2,

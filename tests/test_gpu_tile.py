@@ -86,9 +86,10 @@ def run_case(lf, oracle, N, hm, fwd, rs, split=None):
 
 
 # columns of a row-major (N, hm) array: single-stage tiles of 32 adjacent transforms of 64
-# points, and both stages of the two-stage column plans of 4096 (64 x 64) and 2048 (32 x 64)
-# points
-COLUMN_TILE_CASES = [(64, 40), (4096, 40), (2048, 40)]
+# points (rows 40 elements apart, and 300 apart: not whole 128-byte lines, so the launch is
+# XCD-remapped over a grid padded from 10 to 16 tiles), and both stages of the two-stage
+# column plans of 4096 (64 x 64) and 2048 (32 x 64) points
+COLUMN_TILE_CASES = [(64, 40), (64, 300), (4096, 40), (2048, 40)]
 
 
 @pytest.mark.parametrize("N,hm", COLUMN_TILE_CASES)
