@@ -46,6 +46,7 @@ extern "C" {
 #define MFFT_ERR_UNSUPPORTED 2 /* size/layout outside the engine's domain */
 #define MFFT_ERR_HIP 3         /* a HIP runtime call failed */
 #define MFFT_ERR_ALLOC 4       /* device or host allocation failed */
+#define MFFT_ERR_WORKSPACE 5   /* no intermediate available for a two-stage plan */
 
 typedef struct mfft_plan mfft_plan;
 typedef struct mfft_ragged mfft_ragged;
@@ -102,7 +103,13 @@ int mfft_plan_kind(const mfft_plan *p);
 mfft_plan *mfft_plan_spec(const double *spec, size_t len, unsigned flags);
 
 /* Execute on device pointers (in == out allowed), enqueued on `stream`
- * (a hipStream_t; NULL = the default stream). Replaces lfa_call (Kernels.lean:1203-1204). */
+ * (a hipStream_t; NULL = the default stream). Replaces lfa_call (Kernels.lean:1203-1204).
+ *
+ * Concurrency: a plan owns its device intermediate (two-stage plans) and work buffer
+ * (unfused real-data plans), so executions of ONE plan must not overlap: calls on one
+ * stream are ordered, but one plan must not run on two streams at once (make a plan per
+ * stream, as the Python binding's caches do). Distinct plans run concurrently; planning is
+ * thread-safe. */
 int mfft_execute(const mfft_plan *p, const void *in, void *out, void *stream);
 
 /* Execute with host pointers: copy in, transform, copy out, synchronise. For the
@@ -136,18 +143,28 @@ mfft_plan *mfft_plan_with_radices(int64_t n, int sign, int64_t howmany, const in
                                   int nradix, int stage_split);
 
 /* Ragged batch: `count` transforms of sizes sizes[i], packed back to back (transform i
- * starts at element sum_{j<i} sizes[j]), one direction, executed as one persistent
- * launch per kernel stage from a size-sorted work queue. No counterpart in the reference
- * (it has no batched fast path, bench-leanfftw.c:257). */
+ * starts at element sum_{j<i} sizes[j]), one direction, executed as two persistent launches
+ * from size-sorted work queues: every stage-A and single-stage tile, then every stage-B tile
+ * (the launch boundary orders the stages; a HIP graph replays both as one unit). in != out.
+ * No counterpart in the reference (it has no batched fast path, bench-leanfftw.c:257). */
 mfft_ragged *mfft_plan_ragged(int64_t count, const int64_t *sizes, int sign, unsigned flags);
 int mfft_execute_ragged(const mfft_ragged *p, const void *in, void *out, void *stream);
 /* The schedule a ragged batch uses for its transforms of size n: the radix sequence (return
  * value: its length, -1 if n is not in the batch) and the number of passes of the first
  * kernel stage (*split, 0 for a single stage). */
 int mfft_ragged_class_plan(const mfft_ragged *p, int64_t n, int *radices, int max, int *split);
-/* Synchronise `stream` and report whether the last ragged launch completed every tile
- * (MFFT_SUCCESS) or hit the bound of a stage-B wait (MFFT_ERR_HIP). */
+/* Synchronise `stream`: MFFT_SUCCESS when the enqueued launches completed, else MFFT_ERR_HIP. */
 int mfft_ragged_status(const mfft_ragged *p, void *stream);
+/* Host only (tests): the two work queues of a ragged batch of `count` transforms, built from
+ * explicit class shapes -- per class, ascending distinct sizes, six values: n, stages (1 or
+ * 2), P, Q (n = P Q), tile width of stage A (or of the single stage), of stage B. Writes
+ * five values per tile (class, stage, count, base, sub0) for at most max_tiles tiles, the
+ * first *n_first of them the first launch's, and the transform offsets single-stage tiles
+ * list (base indexes them). Returns the number of tiles, -1 for inconsistent shapes or a
+ * size without a class. No counterpart in the reference. */
+int64_t mfft_ragged_queue(int64_t count, const int64_t *sizes, int64_t nclass,
+                          const int64_t *shapes, int64_t *tiles, int64_t max_tiles,
+                          int64_t *n_first, int64_t *offs, int64_t max_offs);
 void mfft_destroy_ragged(mfft_ragged *p);
 
 /* Fused weighted transform for the differentiable path: out = diag(post) F (diag(pre) in),

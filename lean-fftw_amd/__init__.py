@@ -77,6 +77,10 @@ def lib():
                 "mfft_ragged_status": (ctypes.c_int, [vp, vp]),
                 "mfft_ragged_class_plan": (ctypes.c_int, [vp, i64, ctypes.POINTER(ctypes.c_int),
                                                           ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+                "mfft_ragged_queue": (i64, [i64, ctypes.POINTER(ctypes.c_int64), i64,
+                                            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                            i64, ctypes.POINTER(ctypes.c_int64),
+                                            ctypes.POINTER(ctypes.c_int64), i64]),
                 "mfft_destroy_ragged": (None, [vp]),
                 "mfft_cos_sin_2pi": (None, [ctypes.c_uint64, ctypes.c_uint64, dp, dp]),
                 "mfft_tan_2pi": (ctypes.c_double, [ctypes.c_uint64, ctypes.c_uint64]),
@@ -105,6 +109,30 @@ def _stream_ptr(stream):
     if stream is None:
         stream = torch.cuda.current_stream()
     return ctypes.c_void_p(stream.cuda_stream)
+
+
+# Plans released while a HIP graph is being captured: their device memory is freed at the
+# next release outside a capture (hipFree is not allowed during a capture).
+_deferred: list = []
+
+
+def _capturing() -> bool:
+    try:
+        import torch
+
+        return bool(torch.cuda.is_available() and torch.cuda.is_current_stream_capturing())
+    except Exception:  # pragma: no cover - interpreter shutdown
+        return False
+
+
+def _release(destroy, h) -> None:
+    if _capturing():
+        _deferred.append((destroy, h))
+        return
+    while _deferred:
+        d, old = _deferred.pop()
+        d(old)
+    destroy(h)
 
 
 # ------------------------------------------------------------------------------ twiddles
@@ -185,7 +213,7 @@ class Plan:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h and _lib is not None:
-            _lib.mfft_destroy_plan(h)
+            _release(_lib.mfft_destroy_plan, h)
             self._h = None
 
     @property
@@ -314,7 +342,7 @@ class _CompositePlan:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h and _lib is not None:
-            _lib.mfft_destroy_plan(h)
+            _release(_lib.mfft_destroy_plan, h)
             self._h = None
 
     @property
@@ -401,7 +429,8 @@ class Plan2D(_CompositePlan):
 
 
 class RaggedPlan:
-    """Transforms of mixed sizes packed back to back, one persistent launch per call."""
+    """Transforms of mixed sizes packed back to back: per call, two persistent launches (every
+    stage-A and single-stage tile, then every stage-B tile) from size-sorted queues."""
 
     def __init__(self, sizes, sign=FORWARD, flags=ESTIMATE):
         import numpy as np
@@ -418,7 +447,7 @@ class RaggedPlan:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h and _lib is not None:
-            _lib.mfft_destroy_ragged(h)
+            _release(_lib.mfft_destroy_ragged, h)
             self._h = None
 
     def call(self, inp, out, stream=None):
@@ -433,6 +462,7 @@ class RaggedPlan:
         return out
 
     def status(self, stream=None):
+        """Synchronise the stream; raises if the launches failed."""
         _check(lib().mfft_ragged_status(self._h, _stream_ptr(stream)), "ragged launch")
 
     def class_plan(self, n: int):
@@ -450,8 +480,17 @@ class RaggedPlan:
 _plan_cache: dict = {}
 
 
+def stream_key():
+    """(device, stream) of the current stream: plan caches are per stream, since a plan's
+    intermediate must not be shared by executions that overlap on two streams."""
+    import torch
+
+    s = torch.cuda.current_stream()
+    return (s.device.index, s.cuda_stream)
+
+
 def _cached_plan(n, sign, howmany):
-    key = (n, sign, howmany)
+    key = (n, sign, howmany, stream_key())
     p = _plan_cache.get(key)
     if p is None:
         p = Plan(n, sign, howmany)
@@ -472,6 +511,28 @@ def fft(x, sign: int = FORWARD):
 
 def ifft_unnormalized(x):
     return fft(x, BACKWARD)
+
+
+def ragged_queue(sizes, shapes):
+    """Host only: the two work queues of a ragged batch (mfft_ragged_queue). `shapes` lists
+    (n, stages, P, Q, G_A, G_B) per class in ascending n. Returns (tiles, n_first, offs) with
+    tiles an (ntiles, 5) int64 array of (class, stage, count, base, sub0)."""
+    import numpy as np
+
+    sz = np.ascontiguousarray(sizes, dtype=np.int64)
+    sh = np.ascontiguousarray(shapes, dtype=np.int64).reshape(-1, 6)
+    p64 = ctypes.POINTER(ctypes.c_int64)
+    nf = ctypes.c_int64()
+    n = lib().mfft_ragged_queue(len(sz), sz.ctypes.data_as(p64), len(sh), sh.ctypes.data_as(p64),
+                                None, 0, ctypes.byref(nf), None, 0)
+    if n < 0:
+        raise MfftError("inconsistent ragged queue description")
+    tiles = np.zeros((max(n, 1), 5), dtype=np.int64)
+    offs = np.zeros(max(len(sz), 1), dtype=np.int64)
+    lib().mfft_ragged_queue(len(sz), sz.ctypes.data_as(p64), len(sh), sh.ctypes.data_as(p64),
+                            tiles.ctypes.data_as(p64), n, ctypes.byref(nf),
+                            offs.ctypes.data_as(p64), len(sz))
+    return tiles[:n], int(nf.value), offs
 
 
 from . import workloads  # noqa: E402,F401

@@ -29,7 +29,9 @@ _plans: dict = {}
 
 
 def _plan(n: int, sign: int, howmany: int):
-    key = (n, sign, howmany, torch.cuda.current_device())
+    # one plan per stream: a plan's intermediate must not be shared by executions that
+    # overlap on two streams
+    key = (n, sign, howmany, _lf().stream_key())
     p = _plans.get(key)
     if p is None:
         p = _lf().Plan(n, sign, howmany)

@@ -150,7 +150,7 @@ hipError_t launch_real_part(const double2 *work, double *out, int64_t n, int64_t
 // Fixed-schedule tile kernels (mfft_tile.hip): the kernel for a stage whose shape, radices,
 // lane orders and tile copies match one (1 + its index, 0 if none), and its launch.
 int tile_lookup(const StageArgs &a);
-int xcd_remap(const StageArgs &a);
+int xcd_remap(const StageArgs &a, int G);
 hipError_t launch_tile(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,
                        double2 *out, const double2 *tw, hipStream_t stream);
 
@@ -161,6 +161,19 @@ int tile2d_lookup(int64_t n0, int64_t n1, const StageArgs &col, const StageArgs 
 hipError_t launch_tile2d(bool fwd, int idx, const StageArgs &col, const StageArgs &row,
                          const double2 *in, double2 *out, const double2 *tw0,
                          const double2 *tw1, int64_t howmany, hipStream_t stream);
+
+// A caller-provided intermediate for two-stage plans: `elems` complex values at `ptr`.
+struct Workspace {
+  double2 *ptr;
+  int64_t elems;
+};
+// Execute a plan (any kind) with its own intermediate (ws == nullptr) or with `ws`, in chunks
+// of whole transforms; MFFT_ERR_WORKSPACE when neither is available or `ws` cannot hold one
+// transform. Returns an MFFT_* code.
+int execute_in(const ::mfft_plan *p, const void *in, void *out, const Workspace *ws,
+               hipStream_t st);
+// Complex values of intermediate one transform of the plan needs (0: single stage).
+int64_t workspace_elems(const ::mfft_plan *p);
 
 // Launch one stage over `ntrans` transforms; returns a hipError_t.
 hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,

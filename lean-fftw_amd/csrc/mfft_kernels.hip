@@ -102,8 +102,7 @@ hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const doub
   }();
   (void)attr;
   void *args[] = {(void *)&in, (void *)&out, (void *)&tw, (void *)&a};
-  (void)hipLaunchKernel(k, grid, dim3(a.threads), args, lds, stream);
-  return hipGetLastError();
+  return hipLaunchKernel(k, grid, dim3(a.threads), args, lds, stream);
 }
 
 }  // namespace mfft

@@ -58,6 +58,7 @@ struct mfft_plan {
   int kind = 0;
   mfft_plan *part[2] = {nullptr, nullptr};
   double2 *d_work = nullptr;
+  bool fused_io = false;  // r2c / c2r: the real-data maps fuse into the complex part's kernels
   int64_t n0 = 0, n1 = 0;
   int fused2d = 0;  // 2-D: 1 + index of the single-pass kernel for small arrays, 0 for none
   // an outer batch of single-stage plans: `outer` repetitions of the whole batch, the k-th
@@ -598,8 +599,30 @@ int64_t count_partials(const mfft_plan *p) {
   return nx * p->howmany;  // one row of nx workgroups per transform, over all chunks
 }
 
+// The intermediate of a two-stage plan: the plan's own buffer (chunks of chunk_h transforms),
+// or a caller's region of `elems` complex values, in equal chunks of as many whole
+// transforms as fit (at most 65535, the grid's y extent).
+struct Work {
+  double2 *ptr;
+  int64_t chunk;
+};
+
+int work_for(const mfft_plan *p, const mfft::Workspace *ws, Work &w) {
+  if (!ws) {
+    if (p->stages.size() > 1 && !p->d_scratch) return MFFT_ERR_WORKSPACE;
+    w = {p->d_scratch, p->chunk_h};
+    return MFFT_SUCCESS;
+  }
+  const int64_t hm = std::max<int64_t>(1, p->howmany);
+  const int64_t cmax = std::min<int64_t>({hm, ws->elems / p->n, 65535});
+  if (cmax < 1 || !ws->ptr) return MFFT_ERR_WORKSPACE;
+  const int64_t nchunk = (hm + cmax - 1) / cmax;
+  w = {ws->ptr, (hm + nchunk - 1) / nchunk};
+  return MFFT_SUCCESS;
+}
+
 int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
-        const double *post, double *sumsq, hipStream_t st) {
+        const double *post, double *sumsq, hipStream_t st, const mfft::Workspace *ws = nullptr) {
   if (p->howmany == 0) return MFFT_SUCCESS;
   if (!in || !out) return MFFT_ERR_INVALID;
   const bool fwd = p->sign < 0;
@@ -621,11 +644,13 @@ int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
     }
     return MFFT_SUCCESS;
   }
+  Work w{};
+  if (const int rc = work_for(p, ws, w)) return rc;
   if (p->columns) {
     if (pre || post || sumsq) return MFFT_ERR_UNSUPPORTED;
     const int64_t P = p->stages[0].L, Q = p->stages[1].L;
-    for (int64_t h0 = 0; h0 < p->howmany; h0 += p->chunk_h) {
-      const int64_t hc = std::min(p->chunk_h, p->howmany - h0);
+    for (int64_t h0 = 0; h0 < p->howmany; h0 += w.chunk) {
+      const int64_t hc = std::min(w.chunk, p->howmany - h0);
       StageArgs A = p->stages[0], B = p->stages[1];
       A.nsub = B.nsub = hc;
       A.out_e = hc;  // intermediate [q P + e][transform]
@@ -634,27 +659,23 @@ int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
       B.in_h = hc;
       A.tile = mfft::tile_lookup(A);  // the chunk fixes the tile count
       B.tile = mfft::tile_lookup(B);
-      if (mfft::launch_stage(fwd, A, Q, in + h0 * p->idist, p->d_scratch, p->d_tw, st) !=
-          hipSuccess)
+      if (mfft::launch_stage(fwd, A, Q, in + h0 * p->idist, w.ptr, p->d_tw, st) != hipSuccess)
         return MFFT_ERR_HIP;
-      if (mfft::launch_stage(fwd, B, P, p->d_scratch, out + h0 * p->odist, p->d_tw, st) !=
-          hipSuccess)
+      if (mfft::launch_stage(fwd, B, P, w.ptr, out + h0 * p->odist, p->d_tw, st) != hipSuccess)
         return MFFT_ERR_HIP;
     }
     return MFFT_SUCCESS;
   }
   const int64_t nxb = (p->stages[1].nsub + p->stages[1].G - 1) / p->stages[1].G;
-  for (int64_t h0 = 0; h0 < p->howmany; h0 += p->chunk_h) {
-    const int64_t hc = std::min(p->chunk_h, p->howmany - h0);
+  for (int64_t h0 = 0; h0 < p->howmany; h0 += w.chunk) {
+    const int64_t hc = std::min(w.chunk, p->howmany - h0);
     StageArgs A = p->stages[0], B = p->stages[1];
     A.pre = pre;
     B.post = post;
     B.sumsq = sumsq ? sumsq + h0 * nxb : nullptr;
-    if (mfft::launch_stage(fwd, A, hc, in + h0 * p->idist, p->d_scratch, p->d_tw, st) !=
-        hipSuccess)
+    if (mfft::launch_stage(fwd, A, hc, in + h0 * p->idist, w.ptr, p->d_tw, st) != hipSuccess)
       return MFFT_ERR_HIP;
-    if (mfft::launch_stage(fwd, B, hc, p->d_scratch, out + h0 * p->odist, p->d_tw, st) !=
-        hipSuccess)
+    if (mfft::launch_stage(fwd, B, hc, w.ptr, out + h0 * p->odist, p->d_tw, st) != hipSuccess)
       return MFFT_ERR_HIP;
   }
   return MFFT_SUCCESS;
@@ -677,7 +698,8 @@ bool fusable(const mfft_plan *q, int io_in) {
 // Hermitian halves (2), output real parts (io_out 1) or the first n/2 + 1 values (2). The
 // arithmetic is the complex plan's on the embedded / extended batch, so the bits equal the
 // unfused composition's.
-int run_io(const mfft_plan *q, const void *in, void *out, int io_in, int io_out, hipStream_t st) {
+int run_io(const mfft_plan *q, const void *in, void *out, int io_in, int io_out, hipStream_t st,
+           const mfft::Workspace *ws) {
   const int64_t n = q->n, nh = n / 2 + 1;
   const int64_t din = io_in == 2 ? nh : n, dout = io_out == 2 ? nh : n;  // per transform
   const size_t bin = io_in == 1 ? sizeof(double) : sizeof(double2);
@@ -695,8 +717,10 @@ int run_io(const mfft_plan *q, const void *in, void *out, int io_in, int io_out,
                ? MFFT_SUCCESS
                : MFFT_ERR_HIP;
   }
-  for (int64_t h0 = 0; h0 < q->howmany; h0 += q->chunk_h) {
-    const int64_t hc = std::min(q->chunk_h, q->howmany - h0);
+  Work w{};
+  if (const int rc = work_for(q, ws, w)) return rc;
+  for (int64_t h0 = 0; h0 < q->howmany; h0 += w.chunk) {
+    const int64_t hc = std::min(w.chunk, q->howmany - h0);
     StageArgs A = q->stages[0], B = q->stages[1];
     A.io_in = io_in;
     A.io_n = n;
@@ -706,8 +730,8 @@ int run_io(const mfft_plan *q, const void *in, void *out, int io_in, int io_out,
     B.out_h = dout;
     const auto *src = (const double2 *)((const char *)in + (size_t)h0 * din * bin);
     auto *dst = (double2 *)((char *)out + (size_t)h0 * dout * bout);
-    if (mfft::launch_stage(fwd, A, hc, src, q->d_scratch, q->d_tw, st) != hipSuccess ||
-        mfft::launch_stage(fwd, B, hc, q->d_scratch, dst, q->d_tw, st) != hipSuccess)
+    if (mfft::launch_stage(fwd, A, hc, src, w.ptr, q->d_tw, st) != hipSuccess ||
+        mfft::launch_stage(fwd, B, hc, w.ptr, dst, q->d_tw, st) != hipSuccess)
       return MFFT_ERR_HIP;
   }
   return MFFT_SUCCESS;
@@ -715,12 +739,15 @@ int run_io(const mfft_plan *q, const void *in, void *out, int io_in, int io_out,
 
 // Composite plans: the element maps of mfft_real.hip around the complex parts, or fused
 // into the register kernels' loads and stores where every stage has one.
-int run_composite(const mfft_plan *p, const void *in, void *out, hipStream_t st) {
+int run_composite(const mfft_plan *p, const void *in, void *out, hipStream_t st,
+                  const mfft::Workspace *ws = nullptr) {
   if (!in || !out) return MFFT_ERR_INVALID;
   const int64_t hm = p->howmany;
   if (hm == 0) return MFFT_SUCCESS;
-  if (p->kind == 1 && !p->d_work) return run_io(p->part[0], in, out, 1, 2, st);
-  if (p->kind == 2 && !p->d_work) return run_io(p->part[0], in, out, 2, 1, st);
+  if (p->kind == 1 && p->fused_io) return run_io(p->part[0], in, out, 1, 2, st, ws);
+  if (p->kind == 2 && p->fused_io) return run_io(p->part[0], in, out, 2, 1, st, ws);
+  // the unfused real-data compositions stage the whole batch in the plan's own work buffer
+  if ((p->kind == 1 || p->kind == 2) && (ws || !p->d_work)) return MFFT_ERR_WORKSPACE;
   switch (p->kind) {
     case 1: {  // r2c: embed, forward DFT in place, first n/2 + 1 outputs
       if (mfft::launch_embed_real((const double *)in, p->d_work, p->n, hm, st) != hipSuccess)
@@ -752,10 +779,11 @@ int run_composite(const mfft_plan *p, const void *in, void *out, hipStream_t st)
       const int64_t calls = p->part[0]->outer > 1 ? 1 : hm;
       for (int64_t h = 0; h < calls; ++h) {
         const int rc = run(p->part[0], (const double2 *)in + h * sz, (double2 *)out + h * sz,
-                           nullptr, nullptr, nullptr, st);
+                           nullptr, nullptr, nullptr, st, ws);
         if (rc != MFFT_SUCCESS) return rc;
       }
-      return run(p->part[1], (const double2 *)out, (double2 *)out, nullptr, nullptr, nullptr, st);
+      return run(p->part[1], (const double2 *)out, (double2 *)out, nullptr, nullptr, nullptr, st,
+                 ws);
     }
     default:
       return MFFT_ERR_INVALID;
@@ -789,10 +817,12 @@ mfft_plan *composite_1d(int kind, int64_t n, int64_t howmany, unsigned flags) {
   p->part[0] = inner;
   p->radices = inner->radices;
   p->stage_passes = inner->stage_passes;
-  if (howmany > 0 && !fusable(inner, kind == 1 ? 1 : 2) &&
-      hipMalloc(&p->d_work, (size_t)howmany * n * sizeof(double2)) != hipSuccess) {
-    mfft_destroy_plan(p);
-    return nullptr;
+  p->fused_io = fusable(inner, kind == 1 ? 1 : 2);
+  if (howmany > 0 && !p->fused_io) {
+    if (hipMalloc(&p->d_work, (size_t)howmany * n * sizeof(double2)) != hipSuccess) {
+      mfft_destroy_plan(p);
+      return nullptr;
+    }
   }
   return p;
 }
@@ -1060,6 +1090,24 @@ mfft_plan *plan_many_internal(int64_t n, int sign, int64_t howmany, int64_t istr
   return finish(p, split);
 }
 
+int mfft::execute_in(const mfft_plan *p, const void *in, void *out, const mfft::Workspace *ws,
+                     hipStream_t st) {
+  if (!p) return MFFT_ERR_INVALID;
+  if (p->kind != 0) {
+    if (p->kind != 3 && in == out) return MFFT_ERR_UNSUPPORTED;  // different element types
+    return run_composite(p, in, out, st, ws);
+  }
+  return run(p, (const double2 *)in, (double2 *)out, nullptr, nullptr, nullptr, st, ws);
+}
+
+int64_t mfft::workspace_elems(const mfft_plan *p) {
+  if (!p) return 0;
+  if (p->kind == 0) return p->stages.size() > 1 ? p->n : 0;
+  int64_t e = 0;
+  for (const mfft_plan *q : p->part) e = std::max(e, workspace_elems(q));
+  return e;
+}
+
 int mfft::export_class(const mfft_plan *p, mfft::RClass *c, std::vector<double2> *tw) {
   if (!p || p->howmany != 1 || p->stages.empty() || p->stages.size() > 2) return 1;
   for (const auto &st : p->stages)
@@ -1202,12 +1250,7 @@ mfft_plan *mfft_plan_spec(const double *spec, size_t len, unsigned flags) {
 }
 
 int mfft_execute(const mfft_plan *p, const void *in, void *out, void *stream) {
-  if (!p) return MFFT_ERR_INVALID;
-  if (p->kind != 0) {
-    if (p->kind != 3 && in == out) return MFFT_ERR_UNSUPPORTED;  // different element types
-    return run_composite(p, in, out, (hipStream_t)stream);
-  }
-  return run(p, (const double2 *)in, (double2 *)out, nullptr, nullptr, nullptr, (hipStream_t)stream);
+  return mfft::execute_in(p, in, out, nullptr, (hipStream_t)stream);
 }
 
 mfft_plan *mfft_plan_r2c(int64_t n, int64_t howmany, unsigned flags) {
@@ -1392,6 +1435,8 @@ const char *mfft_error_string(int code) {
     case MFFT_ERR_UNSUPPORTED: return "unsupported problem";
     case MFFT_ERR_HIP: return "HIP runtime error";
     case MFFT_ERR_ALLOC: return "allocation failed";
+    case MFFT_ERR_WORKSPACE:
+      return "no intermediate for a two-stage plan (the workspace cannot hold one transform)";
     default: return "unknown error";
   }
 }

@@ -1,5 +1,5 @@
 // Ragged batches: thousands of transforms of different 13-smooth sizes, packed back to
-// back, in one persistent launch.
+// back, run by persistent launches.
 //
 // The host groups the transforms by size ("classes"; one plan per class, each plan the
 // same Stockham schedule the uniform path uses) and cuts the work into tiles of at most
@@ -9,17 +9,12 @@
 //   * a two-stage class (N = P * Q): stage-A tiles of G_A columns, stage-B tiles of G_B
 //     residue classes, as in the uniform path; stage A writes its intermediate into the
 //     output buffer and stage B transforms it there in place.
-// Tiles are queued largest class first (longest-processing-time order); workgroups take
-// them from the queue with an agent-scope atomic counter until it runs out.
-//
-// Default ("split") schedule: one persistent launch runs every stage-A tile and every
-// single-stage tile, a second runs every stage-B tile; the kernel boundary orders the
-// stages. Alternative ("one", MFFT_RAGGED_MODE=one): a single launch in which the stage-B
-// tiles of a transform are queued a few hundred tiles after its stage-A tiles and wait
-// (bounded spin) until the transform's stage-A counter, raised with release semantics after
-// each stage-A tile, reaches the number of stage-A tiles. Stage-A tiles are always queued
-// before the stage-B tiles of their transform and never wait, so every wait ends; if a spin
-// bound is ever hit the launch records an error flag and still exits.
+// Two queues, each largest class first (longest-processing-time order): the first holds
+// every stage-A tile and then every single-stage tile, the second every stage-B tile. One
+// persistent launch drains each queue (workgroups take tiles with an agent-scope atomic
+// counter until it runs out); the kernel boundary between the two launches orders every
+// stage-B tile after all stage-A tiles, so no workgroup ever waits for another. A HIP graph
+// replays the counter reset and both launches as one unit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,21 +32,16 @@ struct RTile {
   int32_t cls;
   int32_t stage;  // 0: single stage or stage A, 1: stage B
   int32_t count;  // sub-problems (transforms, columns or residues) in the tile
-  int32_t dep;    // two-stage: the transform's counter slot; -1 otherwise
+  int32_t pad;
   int64_t base;   // single stage: index into offs[]; two stage: element offset of the transform
   int64_t sub0;   // two stage: first column / residue
 };
-
-constexpr unsigned kSpinLimit = 1u << 26;
 
 template <bool FWD>
 __global__ __launch_bounds__(WG_THREADS) void ragged_kernel(
     const double2 *in, double2 *out, const double2 *__restrict__ tw,
     const RClass *__restrict__ classes, const RTile *__restrict__ tiles, int ntiles,
-    const int64_t *__restrict__ offs, unsigned *head, unsigned *err,
-    unsigned *deps /* stage-A tiles done, per two-stage transform */,
-    const unsigned *__restrict__ need,
-    int wait /* 0: the stages run as separate launches, no cross-workgroup waits */) {
+    const int64_t *__restrict__ offs, unsigned *head) {
   extern __shared__ double2 lds[];
   __shared__ int s_tile;
   const int tid = threadIdx.x;
@@ -67,21 +57,6 @@ __global__ __launch_bounds__(WG_THREADS) void ragged_kernel(
     const int st = t.stage;
     const int L = c.L[st];
     const int gcount = t.count;
-    if (st == 1 && wait) {
-      if (tid == 0) {
-        unsigned spins = 0;
-        while (__hip_atomic_load(&deps[t.dep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
-               need[t.dep]) {
-          __builtin_amdgcn_s_sleep(2);
-          if (++spins >= kSpinLimit) {
-            __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-          }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // see stage A's stores
-      }
-      __syncthreads();
-    }
     HbmSide hin{}, hout{};
     int gin = 0, gout = 0;
     // load / store policy as in the uniform plans: ordinary loads where the first pass reads
@@ -106,13 +81,6 @@ __global__ __launch_bounds__(WG_THREADS) void ragged_kernel(
                                                             c.G[st], gcount, L, c.npass[st],
                                                             c.pass[st], gin, gout, tw, t.sub0,
                                                             st == 1 ? 1 : 0);
-    if (c.nstage == 2 && st == 0 && wait) {
-      __syncthreads();  // every thread's stores of the tile are issued
-      if (tid == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // publish them
-        __hip_atomic_fetch_add(&deps[t.dep], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
     __syncthreads();
   }
 }
@@ -129,34 +97,102 @@ __global__ void ragged_reset(unsigned *counters, int n) {
 struct mfft_ragged {
   int sign = -1;
   int64_t count = 0, total = 0;
-  int ntiles = 0, ntwo = 0;
-  int n1 = 0, n2 = 0;  // split mode: stage-A and single-stage tiles, then stage-B tiles
-  bool split = true;
+  int n1 = 0, n2 = 0;  // tiles of the first launch (stage A, single stage) and of the second
   int grid = 0;
   size_t lds_bytes = 0;
   mfft::RClass *d_classes = nullptr;
-  mfft::RTile *d_tiles = nullptr;   // one-launch order
-  mfft::RTile *d_tiles1 = nullptr;  // split mode, first launch
-  mfft::RTile *d_tiles2 = nullptr;  // split mode, second launch
+  mfft::RTile *d_tiles1 = nullptr;
+  mfft::RTile *d_tiles2 = nullptr;
   int64_t *d_offs = nullptr;
-  unsigned *d_counters = nullptr;
-  unsigned *d_need = nullptr;
+  unsigned *d_counters = nullptr;  // the two queue heads
   double2 *d_tw = nullptr;
   std::vector<mfft_plan *> plans;  // one per class (twiddle layout source)
   std::map<int64_t, std::pair<std::vector<int>, int>> class_plan;  // size -> (radices, split)
 };
+
+namespace mfft {
+
+// What the queue builder needs of a size class.
+struct RShape {
+  int64_t n;
+  int nstage;
+  int64_t P, Q;  // two stages: N = P * Q
+  int G0, G1;    // sub-problems per tile of each stage
+};
+
+// The two work queues of a ragged batch. `cls` lists the distinct sizes in ascending order
+// (class i has size cls[i].n). Transform i starts at element sum_{j<i} sizes[j]. First queue:
+// the stage-A tiles of the two-stage classes, then the single-stage tiles, each largest class
+// first; second queue: the stage-B tiles, largest class first. Single-stage tiles list their
+// transforms' offsets in `offs`. Returns false if a size has no class.
+bool ragged_queues(int64_t count, const int64_t *sizes, const std::vector<RShape> &cls,
+                   std::vector<RTile> &first, std::vector<RTile> &second,
+                   std::vector<int64_t> &offs) {
+  std::map<int64_t, std::vector<int64_t>> by_size;  // size -> transform offsets
+  std::map<int64_t, int> cls_of;
+  for (size_t i = 0; i < cls.size(); ++i) cls_of[cls[i].n] = (int)i;
+  int64_t off = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    if (!cls_of.count(sizes[i])) return false;
+    by_size[sizes[i]].push_back(off);
+    off += sizes[i];
+  }
+  first.clear();
+  second.clear();
+  offs.clear();
+  std::vector<RTile> single;
+  for (auto it = by_size.rbegin(); it != by_size.rend(); ++it) {
+    const int ci = cls_of[it->first];
+    const RShape &c = cls[ci];
+    const auto &starts = it->second;
+    if (c.nstage == 1) {
+      for (size_t k = 0; k < starts.size(); k += (size_t)c.G0) {
+        RTile t{};
+        t.cls = ci;
+        t.stage = 0;
+        t.count = (int)std::min<size_t>((size_t)c.G0, starts.size() - k);
+        t.base = (int64_t)offs.size();
+        offs.insert(offs.end(), starts.begin() + k, starts.begin() + k + t.count);
+        single.push_back(t);
+      }
+      continue;
+    }
+    for (int64_t start : starts) {
+      for (int64_t c0 = 0; c0 < c.Q; c0 += c.G0) {
+        RTile t{};
+        t.cls = ci;
+        t.stage = 0;
+        t.count = (int)std::min<int64_t>(c.G0, c.Q - c0);
+        t.base = start;
+        t.sub0 = c0;
+        first.push_back(t);
+      }
+      for (int64_t k0 = 0; k0 < c.P; k0 += c.G1) {
+        RTile t{};
+        t.cls = ci;
+        t.stage = 1;
+        t.count = (int)std::min<int64_t>(c.G1, c.P - k0);
+        t.base = start;
+        t.sub0 = k0;
+        second.push_back(t);
+      }
+    }
+  }
+  first.insert(first.end(), single.begin(), single.end());
+  return true;
+}
+
+}  // namespace mfft
 
 namespace {
 
 void free_ragged(mfft_ragged *r) {
   if (!r) return;
   (void)hipFree(r->d_classes);
-  (void)hipFree(r->d_tiles);
   (void)hipFree(r->d_tiles1);
   (void)hipFree(r->d_tiles2);
   (void)hipFree(r->d_offs);
   (void)hipFree(r->d_counters);
-  (void)hipFree(r->d_need);
   (void)hipFree(r->d_tw);
   for (auto *p : r->plans) mfft_destroy_plan(p);
   delete r;
@@ -171,20 +207,20 @@ mfft_ragged *mfft_plan_ragged(int64_t count, const int64_t *sizes, int sign, uns
   auto *r = new mfft_ragged;
   r->sign = sign;
   r->count = count;
-  std::vector<int64_t> offs(count);
-  std::map<int64_t, std::vector<int64_t>> by_size;  // size -> transform indices
-  int64_t off = 0;
+  std::map<int64_t, int> distinct;
   for (int64_t i = 0; i < count; ++i) {
-    offs[i] = off;
-    off += sizes[i];
-    by_size[sizes[i]].push_back(i);
+    if (sizes[i] < 1) {
+      free_ragged(r);
+      return nullptr;
+    }
+    distinct[sizes[i]] = 0;
+    r->total += sizes[i];
   }
-  r->total = off;
 
   std::vector<mfft::RClass> classes;
+  std::vector<mfft::RShape> shapes;
   std::vector<double2> tw;
-  std::map<int64_t, int> cls_of;
-  for (auto &kv : by_size) {
+  for (auto &kv : distinct) {
     mfft_plan *p = plan_many_internal(kv.first, sign, 1, 1, kv.first, 1, kv.first, flags, false);
     if (!p) {
       free_ragged(r);
@@ -202,106 +238,17 @@ mfft_ragged *mfft_plan_ragged(int64_t count, const int64_t *sizes, int sign, uns
       free_ragged(r);
       return nullptr;
     }
-    cls_of[kv.first] = (int)classes.size();
     classes.push_back(c);
+    shapes.push_back({kv.first, c.nstage, c.P, c.Q, c.G[0], c.nstage == 2 ? c.G[1] : 0});
   }
-
-  // tiles, largest class first
-  std::vector<mfft::RTile> single, two;
-  std::vector<int64_t> tile_offs;
-  std::vector<unsigned> need;
-  struct TwoT {
-    std::vector<mfft::RTile> a, b;
-  };
-  std::vector<TwoT> twos;
-  for (auto it = by_size.rbegin(); it != by_size.rend(); ++it) {
-    const int ci = cls_of[it->first];
-    const mfft::RClass &c = classes[ci];
-    const auto &ids = it->second;
-    if (c.nstage == 1) {
-      const int G = c.G[0];
-      for (size_t k = 0; k < ids.size(); k += G) {
-        mfft::RTile t{};
-        t.cls = ci;
-        t.stage = 0;
-        t.count = (int)std::min<size_t>(G, ids.size() - k);
-        t.dep = -1;
-        t.base = (int64_t)tile_offs.size();
-        t.sub0 = 0;
-        for (size_t q = k; q < k + t.count; ++q) tile_offs.push_back(offs[ids[q]]);
-        single.push_back(t);
-      }
-    } else {
-      for (int64_t id : ids) {
-        TwoT tt;
-        const int dep = (int)need.size();
-        unsigned na = 0;
-        for (int64_t c0 = 0; c0 < c.Q; c0 += c.G[0]) {
-          mfft::RTile t{};
-          t.cls = ci;
-          t.stage = 0;
-          t.count = (int)std::min<int64_t>(c.G[0], c.Q - c0);
-          t.dep = dep;
-          t.base = offs[id];
-          t.sub0 = c0;
-          tt.a.push_back(t);
-          ++na;
-        }
-        for (int64_t k0 = 0; k0 < c.P; k0 += c.G[1]) {
-          mfft::RTile t{};
-          t.cls = ci;
-          t.stage = 1;
-          t.count = (int)std::min<int64_t>(c.G[1], c.P - k0);
-          t.dep = dep;
-          t.base = offs[id];
-          t.sub0 = k0;
-          tt.b.push_back(t);
-        }
-        need.push_back(na);
-        twos.push_back(std::move(tt));
-      }
-    }
-  }
-  // Interleave: the stage-B tiles of a transform are queued only after at least LAG
-  // further stage-A tiles, LAG a few times the number of resident workgroups, so that its
-  // stage-A tiles have almost always finished when a workgroup takes its stage-B tiles.
-  int dev0 = 0, ncu0 = 0;
-  (void)hipGetDevice(&dev0);
-  (void)hipDeviceGetAttribute(&ncu0, hipDeviceAttributeMultiprocessorCount, dev0);
-  const size_t LAG = (size_t)std::max(ncu0, 1) * 2 * 3;
-  std::vector<mfft::RTile> order;
-  size_t pending = 0, a_since = 0;  // next transform whose B tiles are pending
-  for (size_t i = 0; i < twos.size(); ++i) {
-    order.insert(order.end(), twos[i].a.begin(), twos[i].a.end());
-    a_since += twos[i].a.size();
-    // release B tiles of earlier transforms once LAG A tiles were queued after them
-    while (pending < i) {
-      size_t after = 0;
-      for (size_t q = pending + 1; q <= i; ++q) after += twos[q].a.size();
-      if (after < LAG) break;
-      order.insert(order.end(), twos[pending].b.begin(), twos[pending].b.end());
-      ++pending;
-    }
-  }
-  // the single-stage tiles never wait: queue them before the remaining B tiles
-  order.insert(order.end(), single.begin(), single.end());
-  for (; pending < twos.size(); ++pending)
-    order.insert(order.end(), twos[pending].b.begin(), twos[pending].b.end());
-  (void)a_since;
-  r->ntiles = (int)order.size();
-  r->ntwo = (int)need.size();
-  // split mode: all stage-A tiles (largest first), then the single-stage tiles; then, in a
-  // second launch, all stage-B tiles
   std::vector<mfft::RTile> first, second;
-  for (auto &tt : twos) first.insert(first.end(), tt.a.begin(), tt.a.end());
-  first.insert(first.end(), single.begin(), single.end());
-  for (auto &tt : twos) second.insert(second.end(), tt.b.begin(), tt.b.end());
+  std::vector<int64_t> tile_offs;
+  if (!mfft::ragged_queues(count, sizes, shapes, first, second, tile_offs)) {
+    free_ragged(r);
+    return nullptr;
+  }
   r->n1 = (int)first.size();
   r->n2 = (int)second.size();
-  {
-    const char *e = std::getenv("MFFT_RAGGED_MODE");
-    r->split = !(e && std::string(e) == "one");
-  }
 
   auto up = [](void **dst, const void *src, size_t bytes) {
     if (bytes == 0) bytes = 16;
@@ -309,14 +256,11 @@ mfft_ragged *mfft_plan_ragged(int64_t count, const int64_t *sizes, int sign, uns
     return src == nullptr || hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
   };
   bool ok = up((void **)&r->d_classes, classes.data(), classes.size() * sizeof(mfft::RClass)) &&
-            up((void **)&r->d_tiles, order.data(), order.size() * sizeof(mfft::RTile)) &&
             up((void **)&r->d_tiles1, first.data(), first.size() * sizeof(mfft::RTile)) &&
             up((void **)&r->d_tiles2, second.data(), second.size() * sizeof(mfft::RTile)) &&
             up((void **)&r->d_offs, tile_offs.empty() ? nullptr : tile_offs.data(),
                tile_offs.size() * sizeof(int64_t)) &&
-            up((void **)&r->d_need, need.empty() ? nullptr : need.data(),
-               need.size() * sizeof(unsigned)) &&
-            up((void **)&r->d_counters, nullptr, (3 + need.size()) * sizeof(unsigned)) &&
+            up((void **)&r->d_counters, nullptr, 2 * sizeof(unsigned)) &&
             up((void **)&r->d_tw, tw.empty() ? nullptr : tw.data(), tw.size() * sizeof(double2));
   if (!ok) {
     free_ragged(r);
@@ -338,49 +282,44 @@ mfft_ragged *mfft_plan_ragged(int64_t count, const int64_t *sizes, int sign, uns
     (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, mfft::kLdsBytes);
   // a few resident workgroups per CU (about 35 KiB of LDS each); workgroups that do not fit
   // start when others finish and take what is left of the queue
-  r->grid = std::max(1, std::min(r->ntiles, std::max(ncu, 1) * 4));
+  r->grid = std::max(1, std::max(ncu, 1) * 4);
   return r;
 }
 
 int mfft_execute_ragged(const mfft_ragged *r, const void *in, void *out, void *stream) {
   if (!r || !in || !out) return MFFT_ERR_INVALID;
   if (in == out) return MFFT_ERR_UNSUPPORTED;  // stage A writes its intermediate into out
-  if (r->ntiles == 0) return MFFT_SUCCESS;
+  if (r->n1 + r->n2 == 0) return MFFT_SUCCESS;
   hipStream_t st = (hipStream_t)stream;
-  const int nc = 3 + r->ntwo;  // [0] head, [1] error, [2] second head, [3 + t] stage-A done
-  unsigned *ctr = r->d_counters;
-  hipLaunchKernelGGL(mfft::ragged_reset, dim3((nc + 255) / 256), dim3(256), 0, st, ctr, nc);
+  unsigned *ctr = r->d_counters;  // [0] first queue's head, [1] second queue's head
+  {
+    int nc = 2;
+    void *args[] = {(void *)&ctr, (void *)&nc};
+    if (hipLaunchKernel((const void *)mfft::ragged_reset, dim3(1), dim3(256), args, 0, st) !=
+        hipSuccess)
+      return MFFT_ERR_HIP;
+  }
   const double2 *x = (const double2 *)in;
   double2 *y = (double2 *)out;
-  auto launch = [&](const mfft::RTile *tiles, int n, unsigned *head, int wait) {
-    if (n == 0) return;
-    const dim3 grid(std::max(1, std::min(n, r->grid))), block(mfft::WG_THREADS);
-    if (r->sign < 0)
-      hipLaunchKernelGGL(mfft::ragged_kernel<true>, grid, block, r->lds_bytes, st, x, y, r->d_tw,
-                         r->d_classes, tiles, n, r->d_offs, head, ctr + 1, ctr + 3, r->d_need,
-                         wait);
-    else
-      hipLaunchKernelGGL(mfft::ragged_kernel<false>, grid, block, r->lds_bytes, st, x, y, r->d_tw,
-                         r->d_classes, tiles, n, r->d_offs, head, ctr + 1, ctr + 3, r->d_need,
-                         wait);
+  const void *k = r->sign < 0 ? (const void *)mfft::ragged_kernel<true>
+                              : (const void *)mfft::ragged_kernel<false>;
+  auto launch = [&](const mfft::RTile *tiles, int n, unsigned *head) {
+    if (n == 0) return hipSuccess;
+    void *args[] = {(void *)&x, (void *)&y, (void *)&r->d_tw, (void *)&r->d_classes,
+                    (void *)&tiles, (void *)&n, (void *)&r->d_offs, (void *)&head};
+    return hipLaunchKernel(k, dim3(std::max(1, std::min(n, r->grid))), dim3(mfft::WG_THREADS),
+                           args, r->lds_bytes, st);
   };
-  if (r->split) {
-    launch(r->d_tiles1, r->n1, ctr, 0);
-    launch(r->d_tiles2, r->n2, ctr + 2, 0);
-  } else {
-    launch(r->d_tiles, r->ntiles, ctr, 1);
-  }
-  return hipGetLastError() == hipSuccess ? MFFT_SUCCESS : MFFT_ERR_HIP;
+  if (launch(r->d_tiles1, r->n1, ctr) != hipSuccess ||
+      launch(r->d_tiles2, r->n2, ctr + 1) != hipSuccess)
+    return MFFT_ERR_HIP;
+  return MFFT_SUCCESS;
 }
 
-// Error flag of the last launch (1 if a stage-B wait hit its bound); synchronises.
+// Synchronise the stream of the last launches and report whether it completed.
 int mfft_ragged_status(const mfft_ragged *r, void *stream) {
   if (!r) return MFFT_ERR_INVALID;
-  unsigned flag = 0;
-  if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess ||
-      hipMemcpy(&flag, r->d_counters + 1, sizeof(unsigned), hipMemcpyDeviceToHost) != hipSuccess)
-    return MFFT_ERR_HIP;
-  return flag ? MFFT_ERR_HIP : MFFT_SUCCESS;
+  return hipStreamSynchronize((hipStream_t)stream) == hipSuccess ? MFFT_SUCCESS : MFFT_ERR_HIP;
 }
 
 int mfft_ragged_class_plan(const mfft_ragged *r, int64_t n, int *radices, int max, int *split) {
@@ -391,6 +330,36 @@ int mfft_ragged_class_plan(const mfft_ragged *r, int64_t n, int *radices, int ma
   for (int i = 0; i < (int)rs.size() && i < max; ++i) radices[i] = rs[i];
   if (split) *split = it->second.second;
   return (int)rs.size();
+}
+
+int64_t mfft_ragged_queue(int64_t count, const int64_t *sizes, int64_t nclass,
+                          const int64_t *shapes, int64_t *tiles, int64_t max_tiles,
+                          int64_t *n_first, int64_t *offs, int64_t max_offs) {
+  if (count < 0 || nclass < 0 || (count && !sizes) || (nclass && !shapes)) return -1;
+  std::vector<mfft::RShape> cls;
+  for (int64_t i = 0; i < nclass; ++i) {
+    const int64_t *s = shapes + 6 * i;
+    if (s[1] < 1 || s[1] > 2 || s[4] < 1 || (s[1] == 2 && (s[5] < 1 || s[2] * s[3] != s[0])))
+      return -1;
+    if (i > 0 && s[0] <= shapes[6 * (i - 1)]) return -1;  // ascending distinct sizes
+    cls.push_back({s[0], (int)s[1], s[2], s[3], (int)s[4], (int)s[5]});
+  }
+  std::vector<mfft::RTile> first, second;
+  std::vector<int64_t> o;
+  if (!mfft::ragged_queues(count, sizes, cls, first, second, o)) return -1;
+  if (n_first) *n_first = (int64_t)first.size();
+  first.insert(first.end(), second.begin(), second.end());
+  for (size_t i = 0; i < first.size() && (int64_t)i < max_tiles && tiles; ++i) {
+    const mfft::RTile &t = first[i];
+    int64_t *d = tiles + 5 * i;
+    d[0] = t.cls;
+    d[1] = t.stage;
+    d[2] = t.count;
+    d[3] = t.base;
+    d[4] = t.sub0;
+  }
+  for (size_t i = 0; i < o.size() && (int64_t)i < max_offs && offs; ++i) offs[i] = o[i];
+  return (int64_t)first.size();
 }
 
 void mfft_destroy_ragged(mfft_ragged *r) { free_ragged(r); }

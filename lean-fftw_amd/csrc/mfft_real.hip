@@ -75,37 +75,37 @@ __global__ __launch_bounds__(kMapThreads) void real_part_kernel(const double2 *_
 hipError_t launch_embed_real(const double *x, double2 *work, int64_t n, int64_t howmany,
                              hipStream_t st) {
   const int64_t total = n * howmany;
-  if (total > 0)
-    hipLaunchKernelGGL(embed_real_kernel, dim3(map_blocks(total)), dim3(kMapThreads), 0, st, x,
-                       work, total);
-  return hipGetLastError();
+  if (total <= 0) return hipSuccess;
+  void *args[] = {(void *)&x, (void *)&work, (void *)&total};
+  return hipLaunchKernel((const void *)embed_real_kernel, dim3(map_blocks(total)), dim3(kMapThreads),
+                         args, 0, st);
 }
 
 hipError_t launch_extract_half(const double2 *work, double2 *out, int64_t n, int64_t howmany,
                                hipStream_t st) {
   const int64_t total = (n / 2 + 1) * howmany;
-  if (total > 0)
-    hipLaunchKernelGGL(extract_half_kernel, dim3(map_blocks(total)), dim3(kMapThreads), 0, st,
-                       work, out, n, total);
-  return hipGetLastError();
+  if (total <= 0) return hipSuccess;
+  void *args[] = {(void *)&work, (void *)&out, (void *)&n, (void *)&total};
+  return hipLaunchKernel((const void *)extract_half_kernel, dim3(map_blocks(total)),
+                         dim3(kMapThreads), args, 0, st);
 }
 
 hipError_t launch_herm_ext(const double2 *in, double2 *work, int64_t n, int64_t howmany,
                            hipStream_t st) {
   const int64_t total = n * howmany;
-  if (total > 0)
-    hipLaunchKernelGGL(herm_ext_kernel, dim3(map_blocks(total)), dim3(kMapThreads), 0, st, in,
-                       work, n, total);
-  return hipGetLastError();
+  if (total <= 0) return hipSuccess;
+  void *args[] = {(void *)&in, (void *)&work, (void *)&n, (void *)&total};
+  return hipLaunchKernel((const void *)herm_ext_kernel, dim3(map_blocks(total)), dim3(kMapThreads),
+                         args, 0, st);
 }
 
 hipError_t launch_real_part(const double2 *work, double *out, int64_t n, int64_t howmany,
                             hipStream_t st) {
   const int64_t total = n * howmany;
-  if (total > 0)
-    hipLaunchKernelGGL(real_part_kernel, dim3(map_blocks(total)), dim3(kMapThreads), 0, st, work,
-                       out, total);
-  return hipGetLastError();
+  if (total <= 0) return hipSuccess;
+  void *args[] = {(void *)&work, (void *)&out, (void *)&total};
+  return hipLaunchKernel((const void *)real_part_kernel, dim3(map_blocks(total)), dim3(kMapThreads),
+                         args, 0, st);
 }
 
 }  // namespace mfft

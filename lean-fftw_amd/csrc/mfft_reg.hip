@@ -590,8 +590,7 @@ hipError_t launch_reg_stage(bool fwd, const StageArgs &a, int64_t ntrans, const 
   const size_t lds = (size_t)a.G * reg_lds_doubles(a.L) * sizeof(double);
   const int bt = a.L / 8 * a.G;
   void *args[] = {(void *)&in, (void *)&out, (void *)&tw, (void *)&b};
-  (void)hipLaunchKernel(k, dim3((unsigned)nx, (unsigned)ntrans), dim3(bt), args, lds, stream);
-  return hipGetLastError();
+  return hipLaunchKernel(k, dim3((unsigned)nx, (unsigned)ntrans), dim3(bt), args, lds, stream);
 }
 
 // Elements per thread: 8 (T = N / 8 <= 1024 threads) up to 8192 points, 16 at 16384.
@@ -627,8 +626,7 @@ hipError_t launch_reg(bool fwd, const StageArgs &a, const double2 *in, double2 *
     }
     void *args[] = {(void *)&in, (void *)&out, (void *)&tw, (void *)&a};
     const int64_t nblk = (a.nsub + G - 1) / G;
-    (void)hipLaunchKernel(k, dim3((unsigned)nblk), dim3(BT), args, lds, stream);
-    return hipGetLastError();
+    return hipLaunchKernel(k, dim3((unsigned)nblk), dim3(BT), args, lds, stream);
   }
   const int E = reg_elems(a.L), T = a.L / E;
   const int BT = T < 256 ? 256 : T, G = BT / T;  // as reg_block<T>()
@@ -662,8 +660,7 @@ hipError_t launch_reg(bool fwd, const StageArgs &a, const double2 *in, double2 *
   }
   void *args[] = {(void *)&in, (void *)&out, (void *)&tw, (void *)&a};
   const int64_t nblk = (a.nsub + G - 1) / G;
-  (void)hipLaunchKernel(k, dim3((unsigned)nblk), dim3(BT), args, lds, stream);
-  return hipGetLastError();
+  return hipLaunchKernel(k, dim3((unsigned)nblk), dim3(BT), args, lds, stream);
 }
 
 }  // namespace mfft

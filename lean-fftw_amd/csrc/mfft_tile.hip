@@ -88,19 +88,19 @@ const TileSchedule kTiles[] = {
 constexpr int kNumTiles = (int)(sizeof(kTiles) / sizeof(kTiles[0]));
 
 // Whether a launch of this stage remaps its workgroups by XCD, for rows that span at least 8
-// tiles: stages whose twiddle base depends on the sub-problem (kb_u != 0, the second stage of
+// tiles of width G: stages whose twiddle base depends on the sub-problem (kb_u != 0, the second stage of
 // a two-stage plan, whose twiddles span the whole table; MI355X: 75600 points 233 -> 218 us,
 // 165375 points 331 -> 313 per 256 MB batch), and stages reading rows of adjacent
 // sub-problems whose stride is not a whole number of 128-byte lines, so that the tiles that
 // share a line meet in one L2 (165375's first stage, rows 7056 B apart: 288 -> 277-279 us;
 // 75600's, 4480 B apart, lost 11 us to it). MFFT_XCD=0 turns it off.
-int xcd_remap(const StageArgs &a) {
+int xcd_remap(const StageArgs &a, int G) {
   static const int on = [] {
     const char *e = std::getenv("MFFT_XCD");
     return e ? std::atoi(e) : 1;
   }();
   const bool split_lines = a.gfast_in && a.in_e % 8 != 0;
-  return on && (a.kb_u != 0 || split_lines) && a.nsub >= 8 * (int64_t)a.G ? 1 : 0;
+  return on && (a.kb_u != 0 || split_lines) && a.nsub >= 8 * (int64_t)G ? 1 : 0;
 }
 
 extern const TileSchedule *const kTileGenPart0, *const kTileGenPart1, *const kTileGenPart2,
@@ -162,7 +162,7 @@ hipError_t launch_tile(bool fwd, const StageArgs &a, int64_t ntrans, const doubl
       (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytesMax);
   }
   StageArgs b = a;
-  b.xcd = xcd_remap(a);
+  b.xcd = xcd_remap(a, t.G);  // the tile's own width, which may differ from the plan's G
   if (a.reg == 1) {
     // in place of the register kernel, whose load / store policy is its own: ordinary loads
     // and non-temporal stores measured best here (ordinary stores 1.2x slower, non-temporal
@@ -174,8 +174,7 @@ hipError_t launch_tile(bool fwd, const StageArgs &a, int64_t ntrans, const doubl
   if (b.xcd) nx = (nx + 7) / 8 * 8;
   const size_t lds = (size_t)stage_lds_elems(t.G, t.L, t.npass, t.io) * sizeof(double2);
   void *args[] = {(void *)&in, (void *)&out, (void *)&tw, (void *)&b};
-  (void)hipLaunchKernel(k, dim3((unsigned)nx, (unsigned)ntrans), dim3(t.NT), args, lds, stream);
-  return hipGetLastError();
+  return hipLaunchKernel(k, dim3((unsigned)nx, (unsigned)ntrans), dim3(t.NT), args, lds, stream);
 }
 
 }  // namespace mfft

@@ -144,8 +144,7 @@ hipError_t launch_tile2d(bool fwd, int idx, const StageArgs &col, const StageArg
   const size_t lds = (size_t)t.n0 * lds_len(t.n1) * sizeof(double2);
   void *args[] = {(void *)&in, (void *)&out, (void *)&tw0, (void *)&tw1, (void *)&col,
                   (void *)&row};
-  (void)hipLaunchKernel(k, dim3((unsigned)howmany), dim3(t.NT), args, lds, stream);
-  return hipGetLastError();
+  return hipLaunchKernel(k, dim3((unsigned)howmany), dim3(t.NT), args, lds, stream);
 }
 
 }  // namespace mfft

@@ -146,6 +146,43 @@ def ref_transform(fwd: bool, rs, x) -> np.ndarray:
     return out.view(np.complex128).reshape(shape)
 
 
+def ref_transform_items(args) -> list:
+    """[ref_transform(fwd, rs, x) for (rs, x) in items], args = (fwd, items): the unit of
+    work of ref_transform_parallel's worker processes."""
+    fwd, items = args
+    return [ref_transform(fwd, rs, x) for rs, x in items]
+
+
+def ref_transform_parallel(fwd: bool, items, workers: int = 8) -> list:
+    """ref_transform over many (radices, x) pairs in `workers` spawned processes (the
+    exact-integer twiddle tables of many distinct sizes dominate the time). Spawned, not
+    forked, so that it is safe in a process that has initialised a GPU."""
+    import multiprocessing as mp
+
+    items = list(items)
+    if workers <= 1 or len(items) < 2:
+        return ref_transform_items((fwd, items))
+    # group by size so each process builds a size's tables once; largest sizes first
+    order = sorted(range(len(items)), key=lambda i: -len(items[i][1]))
+    chunks = [[] for _ in range(workers)]
+    load = [0] * workers
+    for i in order:
+        w = load.index(min(load))
+        chunks[w].append(i)
+        load[w] += len(items[i][1])
+    from concurrent.futures import ProcessPoolExecutor
+
+    # an executor, not a Pool: a worker that dies raises BrokenProcessPool instead of being
+    # replaced forever
+    with ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn")) as ex:
+        parts = list(ex.map(ref_transform_items, [(fwd, [items[i] for i in c]) for c in chunks]))
+    out = [None] * len(items)
+    for c, part in zip(chunks, parts):
+        for i, y in zip(c, part):
+            out[i] = y
+    return out
+
+
 def butterfly_eval(fwd: bool, R: int, tw: bool, s: np.ndarray) -> np.ndarray:
     s = np.ascontiguousarray(s, dtype=np.float64)
     y = np.empty(2 * R, dtype=np.float64)

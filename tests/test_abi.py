@@ -196,3 +196,63 @@ def test_bench2_problem_strings_and_output_format():
     m = re.search(r"setup:\s*([0-9.e+-]+)\s*(ns|us|ms|s),\s*time:\s*([0-9.e+-]+)\s*(ns|us|ms|s)", line)
     assert m and float(m.group(3)) == 1.25
     assert b.main(["-s", "bogus"]) == 2
+
+
+def test_ragged_queue_covers_every_tile_once(lf):
+    """The host queue builder of ragged batches (mfft_ragged_queue, host only): every
+    transform's tiles are queued exactly once, every stage-B tile in the second launch and
+    every stage-A / single-stage tile in the first (so the launch boundary orders each
+    transform's stages), the stage-A tiles ahead of the single-stage ones, classes largest
+    first, tiles within their sub-problem range, single-stage tiles listing the offsets of
+    transforms of their own size."""
+    import numpy as np
+
+    # class shapes: n, stages, P, Q, tile width of stage A (or the single stage), of stage B
+    shapes = [(6, 1, 0, 0, 341, 0), (60, 1, 0, 0, 34, 0), (1000, 1, 0, 0, 2, 0),
+              (4096, 2, 64, 64, 16, 8), (10368, 2, 96, 108, 21, 12), (65520, 2, 240, 273, 8, 8)]
+    rng = np.random.default_rng(3)
+    sizes = rng.choice([s[0] for s in shapes], size=400)
+    tiles, n_first, offs = lf.ragged_queue(sizes, shapes)
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    first, second = tiles[:n_first], tiles[n_first:]
+    assert (second[:, 1] == 1).all() and (first[:, 1] == 0).all()
+    two = [i for i, s in enumerate(shapes) if s[1] == 2]
+    is_two = np.isin(first[:, 0], two)
+    # stage-A tiles first, then single-stage tiles
+    assert not (np.diff(is_two.astype(int)) > 0).any()
+    # largest class first within each run
+    for part in (first[is_two], first[~is_two], second):
+        assert (np.diff(part[:, 0]) <= 0).all()
+    seen_single = []
+    cover = {}
+    for cls, stage, count, base, sub0 in tiles:
+        n, ns, P, Q, g0, g1 = shapes[cls]
+        if ns == 1:
+            assert 1 <= count <= g0
+            tr = offs[base:base + count]
+            assert all(sizes[np.searchsorted(starts, o)] == n for o in tr)
+            seen_single.extend(int(o) for o in tr)
+        else:
+            lim, width = (Q, g0) if stage == 0 else (P, g1)
+            assert 0 <= sub0 < lim and 1 <= count <= width and sub0 + count <= lim
+            assert base in set(starts[sizes == n])
+            cover.setdefault((int(base), int(stage)), []).append((int(sub0), int(count)))
+    single_starts = sorted(int(o) for o, n in zip(starts, sizes) if shapes_n_stages(shapes, n) == 1)
+    assert sorted(seen_single) == single_starts
+    for o, n in zip(starts, sizes):
+        ns, P, Q = [(s[1], s[2], s[3]) for s in shapes if s[0] == n][0]
+        if ns != 2:
+            continue
+        for stage, lim in ((0, Q), (1, P)):
+            runs = sorted(cover[(int(o), stage)])
+            pos = 0
+            for a, c in runs:  # contiguous, no overlap, complete
+                assert a == pos
+                pos += c
+            assert pos == lim
+    with pytest.raises(lf.MfftError):
+        lf.ragged_queue([6, 7], shapes)  # a size without a class
+
+
+def shapes_n_stages(shapes, n):
+    return [s[1] for s in shapes if s[0] == n][0]

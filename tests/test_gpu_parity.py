@@ -237,9 +237,35 @@ def test_deterministic_rerun(lf, oracle):
 
 
 # every size with a published reference L2: 2^1..2^18 and the 18 cross-checked 13-smooth
-# sizes (results/repro-2026-11-04/xcheck-patient.jsonl), config 3's published ones included
+# sizes (results/repro-2026-11-04/xcheck-patient.jsonl), config 3's published ones included;
+# and config 3's unpublished sizes 60, 360 and 2100, whose bar is 2x the nearest published
+# size's (log distance: 64, 504 and 2048)
 ACCURACY_SIZES = [2 ** k for k in range(1, 19)] + [6, 9, 12, 15, 18, 24, 36, 80, 108, 210, 504,
-                                                   1000, 1960, 4725, 10368, 27000, 75600, 165375]
+                                                   1000, 1960, 4725, 10368, 27000, 75600, 165375,
+                                                   60, 360, 2100]
+
+
+def reference_l2(N):
+    """(reference L2, the size it was published for). Published sizes: the larger of
+    libbench2's --accuracy figure (emr-i12) and the cross-check against long-double FFTW
+    (xcheck); unpublished ones: the nearest published size in log distance, over those two
+    sets and the non-power-of-two --accuracy run (spr-nonpow2-i9)."""
+    import json
+    import os
+
+    from conftest import GOLDEN
+
+    acc = json.load(open(os.path.join(GOLDEN, "accuracy_ref.json")))
+    pub = {}
+    for k, f in (("pow2", "fwd_l2"), ("xcheck", "lean_l2")):
+        for n, v in acc[k].items():
+            pub.setdefault(int(n), []).append(v[f])
+    if N in pub:
+        return max(pub[N]), N
+    for n, v in acc["nonpow2"]["lean-i8"].items():
+        pub.setdefault(int(n), []).append(v["fwd_l2"])
+    near = min(pub, key=lambda m: (abs(math.log(m / N)), m))
+    return max(pub[near]), near
 
 
 @pytest.mark.parametrize("N", ACCURACY_SIZES)
@@ -247,17 +273,9 @@ def test_accuracy_within_2x_reference(lf, oracle, N):
     import json
     import os
 
-    from conftest import GOLDEN
-
-    acc = json.load(open(os.path.join(GOLDEN, "accuracy_ref.json")))
-    # two published measurements: libbench2 --accuracy (emr-i12, its own inputs) and the
-    # cross-check against long-double FFTW (xcheck); where both exist the bar takes the
-    # larger (at N = 4 libbench2's inputs make the transform exact, 3e-44, the cross-check
-    # reads 5.6e-17)
-    refs = [acc[k][str(N)][f] for k, f in (("pow2", "fwd_l2"), ("xcheck", "lean_l2"))
-            if str(N) in acc[k]]
-    assert refs
-    ref = max(refs)
+    # at N = 4 libbench2's inputs make the transform exact (3e-44), the cross-check reads
+    # 5.6e-17: the bar takes the larger published figure
+    ref, ref_n = reference_l2(N)
     rounds = 4 if N >= 65536 else 10
     x = oracle.lcg_input(N, seed=4242 + N, howmany=rounds)
     plan = lf.Plan(N, lf.FORWARD, rounds)
@@ -267,14 +285,20 @@ def test_accuracy_within_2x_reference(lf, oracle, N):
     if log:  # one JSON line per size, for the per-size table of the design notes
         with open(log, "a") as f:
             f.write(json.dumps({"N": N, "radices": plan.radices, "l2_mean": float(err.mean()),
-                                "l2_max": float(err.max()), "reference_l2": ref}) + "\n")
+                                "l2_max": float(err.max()), "reference_l2": ref,
+                                "reference_n": ref_n}) + "\n")
     # libbench2's --accuracy averages the relative L1/L2 errors over its rounds (and takes
     # the max of L-inf); the bar is 2x the reference's published figure for the mean, with
     # every single round within 3x.
     # (N = 2 and 4 are exact in binary64: the reference reports 0 and 3e-44, and what is left
     # here is the binary128 comparison's own rounding, so the bar has a floor of 1e-32)
-    assert err.mean() <= max(2 * ref, 1e-32), (N, err.mean(), ref)
-    assert err.max() <= max(3 * ref, 1e-32), (N, err.max(), ref)
+    assert err.mean() <= max(2 * ref, 1e-32), (N, err.mean(), ref, ref_n)
+    assert err.max() <= max(3 * ref, 1e-32), (N, err.max(), ref, ref_n)
+    # and a floor: a binary64 transform of these inputs has an error of the order of the
+    # reference's (0.74-1.3x of it at every size measured), so a metric reading far below it
+    # is itself broken
+    if N >= 8:
+        assert err.mean() >= 0.2 * ref, (N, err.mean(), ref, ref_n)
 
 
 # ------------------------------------------------------------------ full-size properties
@@ -317,3 +341,60 @@ def test_full_size_pow2_properties(lf, oracle, k):
     xs = x[rows].cpu().numpy()
     ref = oracle.ref_transform(True, fwd.radices, xs)
     assert bits_equal(y[rows].cpu().numpy(), ref), (N, fwd.radices, fwd.stages)
+
+
+CONFIG3 = [6, 60, 360, 1000, 2100, 10368, 75600, 165375]
+
+
+@pytest.mark.parametrize("N", CONFIG3)
+def test_full_size_mixed_properties(lf, oracle, N):
+    """Config of the mixed-radix batch: N 13-smooth, batch ceil(2^24 / N) (at least 256 MB,
+    the XCD-remapped and padded grids of the two-stage sizes at their real extent). Round
+    trip backward(forward(x)) = N x, Parseval, and the first, middle and last transforms bit
+    for bit against the oracle."""
+    hm = -(-(1 << 24) // N)
+    g = torch.Generator(device=DEV).manual_seed(N)
+    x = (torch.rand(hm, N, dtype=torch.float64, device=DEV, generator=g) - 0.5) + 1j * (
+        torch.rand(hm, N, dtype=torch.float64, device=DEV, generator=g) - 0.5)
+    fwd = lf.Plan(N, lf.FORWARD, hm)
+    bwd = lf.Plan(N, lf.BACKWARD, hm)
+    y = torch.empty_like(x)
+    fwd.call(x, y)
+    z = torch.empty_like(x)
+    bwd.call(y, z)
+    torch.cuda.synchronize()
+    err = ((z / N - x).abs().max() / x.abs().max()).item()
+    assert err < 1e-14 * math.log2(N) + 1e-15
+    ex = (x.abs() ** 2).sum().item()
+    ey = (y.abs() ** 2).sum().item()
+    assert abs(ey - N * ex) <= 1e-12 * N * ex
+    rows = [0, hm // 2, hm - 1]
+    ref = oracle.ref_transform(True, fwd.radices, x[rows].cpu().numpy())
+    assert bits_equal(y[rows].cpu().numpy(), ref), (N, fwd.radices, fwd.stages)
+    refb = oracle.ref_transform(False, bwd.radices, y[rows].cpu().numpy())
+    assert bits_equal(z[rows].cpu().numpy(), refb), (N, bwd.radices, bwd.stages)
+
+
+def test_fft_on_two_streams_same_shape(lf):
+    """lf.fft keeps a plan per stream, so one shape running on two streams at once (two-stage
+    sizes own an intermediate) gives each stream its own plan and the eager results."""
+    N, hm = 65536, 64
+    g = torch.Generator(device=DEV).manual_seed(9)
+    xs = [torch.complex(torch.rand(hm, N, dtype=torch.float64, device=DEV, generator=g),
+                        torch.rand(hm, N, dtype=torch.float64, device=DEV, generator=g))
+          for _ in range(2)]
+    refs = [lf.fft(x) for x in xs]
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs = []
+    for _ in range(3):
+        outs = []
+        for s, x in zip(streams, xs):
+            s.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(s):
+                outs.append(lf.fft(x))
+        for s in streams:
+            torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        for o, r in zip(outs, refs):
+            assert torch.equal(o, r)

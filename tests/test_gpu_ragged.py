@@ -1,5 +1,6 @@
-"""Ragged batches in one persistent launch: every transform bit-identical to the oracle's
-refTransform with the radix sequence the uniform path uses for that size."""
+"""Ragged batches (two persistent launches from size-sorted queues): every transform
+bit-identical to the oracle's refTransform with the radix sequence the uniform path uses for
+that size."""
 
 import numpy as np
 import pytest
@@ -52,8 +53,8 @@ def test_ragged_two_stage_sizes(lf, oracle):
 
 
 def test_ragged_benchmark_draw_prefix(lf, oracle):
-    """The first 600 sizes of the benchmark's log-uniform draw, run twice (the queue and the
-    dependency counters are reset by every launch)."""
+    """The first 600 sizes of the benchmark's log-uniform draw, run twice (the queue heads
+    are reset by every call)."""
     w = load_product().workloads
     sizes = w.ragged_sizes()[:600]
     check_ragged(lf, oracle, sizes, -1, 4, reps=2)
@@ -84,6 +85,36 @@ def test_ragged_full_draw_against_uniform(lf):
         off += n
     torch.cuda.synchronize()
     assert torch.equal(out.view(torch.float64), ref.view(torch.float64))
+
+
+def test_ragged_full_draw_against_oracle(lf, oracle):
+    """The whole 10,000-transform benchmark draw against the oracle: the first transform of
+    every one of its 1,424 sizes and every 50th transform, bit for bit (the oracle runs in 8
+    worker processes: its exact-integer twiddle tables for 1,424 sizes take minutes on one
+    core)."""
+    w = load_product().workloads
+    sizes = [int(n) for n in w.ragged_sizes()]
+    total = sum(sizes)
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    g = torch.Generator(device=DEV).manual_seed(6)
+    x = torch.complex(torch.rand(total, dtype=torch.float64, device=DEV, generator=g) - 0.5,
+                      torch.rand(total, dtype=torch.float64, device=DEV, generator=g) - 0.5)
+    out = torch.empty_like(x)
+    rp = lf.RaggedPlan(sizes, -1)
+    rp.call(x, out)
+    rp.status()
+    first = {}
+    for i, n in enumerate(sizes):
+        first.setdefault(n, i)
+    pick = sorted(set(first.values()) | set(range(0, len(sizes), 50)))
+    xs = x.cpu().numpy()
+    got = out.cpu().numpy()
+    items = [(rp.class_plan(sizes[i])[0], xs[starts[i]:starts[i] + sizes[i]]) for i in pick]
+    refs = oracle.ref_transform_parallel(True, items, workers=8)
+    bad = [(i, sizes[i]) for i, r in zip(pick, refs)
+           if not np.array_equal(got[starts[i]:starts[i] + sizes[i]].view(np.uint64),
+                                 r.view(np.uint64))]
+    assert len(pick) >= 1424 + 150 and not bad, bad[:10]
 
 
 def test_ragged_rejects_inplace_and_bad_sizes(lf):

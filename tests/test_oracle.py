@@ -290,10 +290,69 @@ def test_against_quad_dft(oracle, N, rs):
 
 
 def test_quad_dft_vs_mpmath(oracle):
+    """The binary128 DFT against a 200-bit brute-force DFT: the prime factors 2, 3, 5, 7,
+    11 and 13, one- to four-level recursions (143 = 11 x 13, 210 = 2 x 3 x 5 x 7,
+    390 = 2 x 3 x 5 x 13), both directions."""
     import mpmath
 
     mpmath.mp.prec = 200
-    for N in (6, 16, 35):
+    for N, fwd in ((6, True), (16, True), (35, True), (143, True), (143, False), (210, True),
+                   (390, False)):
+        x = oracle.lcg_input(N, seed=99 + N)
+        hi, lo = oracle.mp_dft(fwd, x)
+        sgn = -1 if fwd else 1
+        w = [mpmath.expjpi(sgn * mpmath.mpf(2 * m) / N) for m in range(N)]
+        xs = [mpmath.mpc(v.real, v.imag) for v in x]
+        for k in range(N):
+            acc = mpmath.fsum(xs[j] * w[(j * k) % N] for j in range(N))
+            got = mpmath.mpc(hi[k].real, hi[k].imag) + mpmath.mpc(lo[k].real, lo[k].imag)
+            assert abs(got - acc) < mpmath.mpf(2) ** -100, (N, fwd, k)
+
+
+def test_rel_l2_reads_back_a_known_error(oracle):
+    """rel_l2 = ||y - Y||_2 / ||Y||_2 pinned from both sides.
+
+    Closed forms that need no DFT value: the impulse transforms to all ones, so y = ones
+    reads 0, y = 0 reads 1, y = -Y reads 2, y = (1 + e) ones reads e, one entry off by d
+    reads |d| / sqrt(N). Then y = Y + delta with a chosen ||delta|| / ||Y|| from 1e-6 to
+    1e-15 reads back the ratio computed independently at 200 bits from the stored doubles
+    (a missing square root, a 1/N, a dropped denominator or a sign error fails)."""
+    import mpmath
+
+    for N in (8, 60, 1000):
+        imp = np.zeros(N, np.complex128)
+        imp[0] = 1
+        x = imp[None, :]
+        ones = np.ones((1, N), np.complex128)
+        assert oracle.rel_l2(True, x, ones)[0] == 0.0
+        assert oracle.rel_l2(True, x, np.zeros_like(ones))[0] == 1.0
+        assert oracle.rel_l2(True, x, -ones)[0] == 2.0
+        e = 2.0 ** -30
+        assert oracle.rel_l2(True, x, ones * (1 + e))[0] == e
+        y = ones.copy()
+        y[0, N // 3] += 0.5j
+        assert abs(oracle.rel_l2(False, x, y)[0] - 0.5 / math.sqrt(N)) <= 1e-16
+    mpmath.mp.prec = 200
+    rng = np.random.default_rng(11)
+    for N in (16, 60, 143):
+        x = oracle.lcg_input(N, seed=31 + N)
+        hi, lo = oracle.mp_dft(True, x)
+        for eps in (1e-6, 1e-10, 1e-14, 1e-15):
+            d = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+            d *= eps * np.linalg.norm(hi) / np.linalg.norm(d)
+            y = hi + d
+            got = oracle.rel_l2(True, x[None, :], y[None, :])[0]
+            num = den = mpmath.mpf(0)
+            for k in range(N):
+                yr = mpmath.mpf(hi[k].real) + mpmath.mpf(lo[k].real)
+                yi = mpmath.mpf(hi[k].imag) + mpmath.mpf(lo[k].imag)
+                dr, di = mpmath.mpf(y[k].real) - yr, mpmath.mpf(y[k].imag) - yi
+                num += dr * dr + di * di
+                den += yr * yr + yi * yi
+            want = float(mpmath.sqrt(num / den))
+            assert abs(got - want) <= 1e-9 * want, (N, eps, got, want)
+            # storing y = hi + delta in binary64 moves it by at most 2^-53 |y| per component
+            assert abs(want - eps) <= 0.15 * eps, (N, eps, want)
         x = oracle.lcg_input(N, seed=99 + N)
         hi, lo = oracle.mp_dft(True, x)
         for k in range(N):
