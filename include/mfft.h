@@ -112,6 +112,20 @@ mfft_plan *mfft_plan_spec(const double *spec, size_t len, unsigned flags);
  * thread-safe. */
 int mfft_execute(const mfft_plan *p, const void *in, void *out, void *stream);
 
+/* As mfft_execute, but the intermediate of two-stage plans is the library's static device
+ * scratch -- a file-scope array of MFFT_STATIC_SCRATCH_BYTES (64 MiB) per GPU, held until the
+ * process ends -- instead of the plan's own buffer; a batch runs in chunks of as many whole
+ * transforms as fit. Allocates nothing (no hipMalloc / hipFree, no allocating library
+ * routine). For callers that cannot allocate: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid allocation. MFFT_ERR_WORKSPACE if one transform's
+ * intermediate exceeds the scratch (n above 4,194,304 points); MFFT_ERR_UNSUPPORTED for
+ * real-data plans whose maps do not fuse into their kernels (they stage the whole batch in a
+ * work buffer). Every call on a GPU shares that GPU's scratch: calls that overlap on one GPU
+ * (two streams, two threads) race on it. Nothing in it needs to start at zero (stage A
+ * writes every element stage B reads). No counterpart in the reference. */
+#define MFFT_STATIC_SCRATCH_BYTES (64u << 20)
+int mfft_execute_static(const mfft_plan *p, const void *in, void *out, void *stream);
+
 /* Execute with host pointers: copy in, transform, copy out, synchronise. For the
  * libbench2-shaped backend (bench-leanfftw.c:281-291) and host-array callers. */
 int mfft_execute_host(const mfft_plan *p, const double *in, double *out);

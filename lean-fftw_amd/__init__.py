@@ -63,6 +63,7 @@ def lib():
                                                 ctypes.c_int, ctypes.c_int]),
                 "mfft_execute": (ctypes.c_int, [vp, vp, vp, vp]),
                 "mfft_execute_host": (ctypes.c_int, [vp, dp, dp]),
+                "mfft_execute_static": (ctypes.c_int, [vp, vp, vp, vp]),
                 "mfft_execute_spec": (ctypes.c_int, [vp, vp, vp, vp]),
                 "mfft_execute_weighted": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
                 "mfft_execute_weighted_sumsq": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp]),
@@ -245,6 +246,16 @@ class Plan:
         _check(lib().mfft_execute(self._h, ctypes.c_void_p(inp.data_ptr()),
                                   ctypes.c_void_p(out.data_ptr()), _stream_ptr(stream)),
                "mfft_execute")
+        return out
+
+    def call_static(self, inp, out, stream=None):
+        """As call, with the library's static device scratch as the two-stage intermediate
+        (mfft_execute_static: allocates nothing; calls overlapping on one GPU race)."""
+        self._check_tensor(inp, "input")
+        self._check_tensor(out, "output")
+        _check(lib().mfft_execute_static(self._h, ctypes.c_void_p(inp.data_ptr()),
+                                         ctypes.c_void_p(out.data_ptr()), _stream_ptr(stream)),
+               "mfft_execute_static")
         return out
 
     def call_weighted(self, inp, out, pre=None, post=None, stream=None):

@@ -35,6 +35,7 @@ SOURCES = [
     *((f"mfft_tile_gen{i}.hip", True) for i in range(8)),
     ("mfft_real.hip", True),
     ("mfft_ragged.hip", True),
+    ("mfft_static.hip", True),
     ("mfft_plan.cpp", False),
     ("mfft_trig.cpp", False),
 ]

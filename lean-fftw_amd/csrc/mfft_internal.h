@@ -174,6 +174,8 @@ int execute_in(const ::mfft_plan *p, const void *in, void *out, const Workspace 
                hipStream_t st);
 // Complex values of intermediate one transform of the plan needs (0: single stage).
 int64_t workspace_elems(const ::mfft_plan *p);
+// The device a plan was made on.
+int plan_device(const ::mfft_plan *p, int *dev);
 
 // Launch one stage over `ntrans` transforms; returns a hipError_t.
 hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,

@@ -747,7 +747,8 @@ int run_composite(const mfft_plan *p, const void *in, void *out, hipStream_t st,
   if (p->kind == 1 && p->fused_io) return run_io(p->part[0], in, out, 1, 2, st, ws);
   if (p->kind == 2 && p->fused_io) return run_io(p->part[0], in, out, 2, 1, st, ws);
   // the unfused real-data compositions stage the whole batch in the plan's own work buffer
-  if ((p->kind == 1 || p->kind == 2) && (ws || !p->d_work)) return MFFT_ERR_WORKSPACE;
+  if ((p->kind == 1 || p->kind == 2) && ws) return MFFT_ERR_UNSUPPORTED;
+  if ((p->kind == 1 || p->kind == 2) && !p->d_work) return MFFT_ERR_WORKSPACE;
   switch (p->kind) {
     case 1: {  // r2c: embed, forward DFT in place, first n/2 + 1 outputs
       if (mfft::launch_embed_real((const double *)in, p->d_work, p->n, hm, st) != hipSuccess)
@@ -1098,6 +1099,12 @@ int mfft::execute_in(const mfft_plan *p, const void *in, void *out, const mfft::
     return run_composite(p, in, out, st, ws);
   }
   return run(p, (const double2 *)in, (double2 *)out, nullptr, nullptr, nullptr, st, ws);
+}
+
+int mfft::plan_device(const mfft_plan *p, int *dev) {
+  if (!p || !dev) return MFFT_ERR_INVALID;
+  *dev = p->kind == 0 ? p->device : p->part[0]->device;  // composites: their parts' device
+  return MFFT_SUCCESS;
 }
 
 int64_t mfft::workspace_elems(const mfft_plan *p) {
