@@ -11,7 +11,10 @@ transform: 16N read + 16N written) of the whole step divided by the step time, G
 
 Prints one JSON line. `roofline` is measured live with HIP events on the stream the
 kernels run on; `cpu_baseline` times the oracle (the C restatement of the reference's
-refTransform, one thread) on a bounded sample of the same workload.
+refTransform, one pinned core) with libbench2's timing rule, beside the reference's published
+single-core times. The default line also carries the metric's other two parts: the ragged
+10,000-transform batch (`ragged`) and the differentiable transform's forward + backward over
+forward (`autograd`).
 """
 
 from __future__ import annotations
@@ -68,46 +71,93 @@ def time_steps(torch, step, steps, warmup):
 
 # ------------------------------------------------------------------------------ CPU baseline
 
+# The reference's published single-core times (ns per transform, one transform, out of place,
+# forward): its assembly engine i12 on Emerald Rapids at 2.1 GHz, one core pinned, median of
+# 10 process runs of libbench2's min-of-8 (the reference's docs/RESULTS.md:70-89,
+# results/emr-i12/summary.txt).
+REF_PUBLISHED_NS = {2: 0.99, 4: 1.605, 8: 3.01, 16: 7.245, 32: 17.17, 64: 40.4, 128: 96.16,
+                    256: 217.1, 512: 420.9, 1024: 1075.0, 2048: 2405.0, 4096: 6200.0,
+                    8192: 12920.0, 16384: 29490.0, 32768: 70270.0, 65536: 208300.0,
+                    131072: 647700.0, 262144: 1355000.0}
 
-def cpu_baseline_sweep(sizes, plans_radices, seconds):
-    """The oracle (C restatement of refTransform, one thread) on a bounded sample of each
-    size of the sweep: `pts` points per size, min of 3 runs, scaled to batch 2^24/N."""
+
+def host_cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def libbench2_time(run, min_seconds=0.01, samples=8):
+    """libbench2's timing rule: double the repetitions until one timing takes at least
+    10 ms, then the minimum of 8 timings; seconds per repetition."""
+    reps = 1
+    while True:
+        t0 = time.perf_counter()
+        run(reps)
+        if time.perf_counter() - t0 >= min_seconds:
+            break
+        reps *= 2
+    best = float("inf")
+    for _ in range(samples):
+        t0 = time.perf_counter()
+        run(reps)
+        best = min(best, time.perf_counter() - t0)
+    return best / reps
+
+
+def cpu_baseline_sweep(sizes, plans_radices, budget_s):
+    """The oracle -- the C restatement of the reference's refTransform with the GPU plans'
+    radices -- on one pinned core, timed per size with libbench2's rule on one transform
+    repeated in C (as libbench2 drives the reference), and scaled to the sweep's batch of
+    2^24 / N transforms per size. The reference's own published per-size times (another
+    host) are given beside it, aggregated the same way."""
     sys.path.insert(0, ROOT)
     import numpy as np
 
     import oracle
 
     oracle.build()
-    pts = 1 << 22
-    # calibrate the sample so that the whole leg takes about `seconds`
-    total_bytes = 0.0
-    total_time = 0.0
+    old = os.sched_getaffinity(0)
+    core = min(old)
+    os.sched_setaffinity(0, {core})
+    per, total_b, total_t, ref_t = [], 0.0, 0.0, 0.0
     t_start = time.perf_counter()
-    for N in sizes:
-        rp = oracle.RefPlan(True, N, plans_radices[N])
-        hm = max(1, pts // N)
-        x = oracle.lcg_input(N, seed=oracle.LCG_SEED ^ N, howmany=hm)
-        x = x.reshape(hm, N)
-        out = np.empty_like(x)
-        best = float("inf")
-        for _ in range(3):
-            t0 = time.perf_counter()
-            rp.execute(x, out)
-            best = min(best, time.perf_counter() - t0)
-        full = best * ((1 << 24) // N) / hm
-        total_time += full
-        total_bytes += 32.0 * (1 << 24)
-        if time.perf_counter() - t_start > 4 * seconds:
-            break
+    try:
+        for N in sizes:
+            rp = oracle.RefPlan(True, N, plans_radices[N])
+            x = oracle.lcg_input(N, seed=oracle.LCG_SEED ^ N)
+            out = np.empty_like(x)
+            t = libbench2_time(lambda reps: rp.repeat(x, out, reps))
+            batch = (1 << 24) // N
+            total_t += t * batch
+            total_b += 32.0 * N * batch
+            ref_t += REF_PUBLISHED_NS[N] * 1e-9 * batch
+            per.append({"N": N, "ns": round(t * 1e9, 2), "ref_published_ns": REF_PUBLISHED_NS[N]})
+            if time.perf_counter() - t_start > budget_s:
+                break
+    finally:
+        os.sched_setaffinity(0, old)
+    done = len(per)
     return {
-        "value": round(total_bytes / total_time / 1e9, 3),
+        "value": round(total_b / total_t / 1e9, 3),
         "unit": "GB/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"oracle refTransform (C, gcc -O2, one thread) with the GPU plans' radices, "
-                  f"{pts} points per size for each of the {len(sizes)} sweep sizes, min of 3, "
-                  f"time scaled to batch 2^24/N; host {platform.processor() or platform.machine()}, "
+        "sample": f"oracle refTransform (C: gcc {' '.join(oracle.CFLAGS)}) with the GPU plans' "
+                  f"radices, one transform per size for {done} of the {len(sizes)} sweep sizes, "
+                  f"pinned to core {core}, libbench2's rule (repetitions doubled to >= 10 ms, "
+                  f"min of 8), scaled to batch 2^24/N; host {host_cpu_model()}, "
                   f"{os.cpu_count()} logical CPUs",
+        "reference_published": {
+            "value": round(total_b / ref_t / 1e9, 3), "unit": "GB/s",
+            "host": "reference assembly engine i12, Emerald Rapids 2.1 GHz, 1 core (published)",
+            "aggregated": "its per-size times of one transform scaled to batch 2^24/N, as above"},
+        "per_size": per,
     }
 
 
@@ -131,7 +181,7 @@ def bench_sweep(args, torch, lf):
             plans[N].call(xi, yi)
 
     t = time_steps(torch, step, args.steps, args.warmup)
-    # per-size timing (events around each batched transform), for the roofline breakdown
+    # per-size timing (events around back-to-back batched transforms of one size)
     per = []
     for N in sizes:
         xi, yi = views[N]
@@ -152,14 +202,18 @@ def bench_sweep(args, torch, lf):
     bytes_step = 32.0 * total * len(sizes)
     value = bytes_step / t / 1e9
     # HBM traffic per step from the committed rocprofv3 PMC summary (FETCH_SIZE, WRITE_SIZE,
-    # each scaled by a plain copy kernel), when it describes this sweep
-    traffic = None
+    # each scaled by a plain copy kernel), used only when its plans are this run's plans
+    traffic, traffic_src = None, None
     tpath = os.path.join(ROOT, "profiles", "traffic_pow2_sweep.json")
     if os.path.exists(tpath) and len(sizes) == 18:
         with open(tpath) as f:
-            traffic = round(json.load(f)["step_traffic_bytes"])
-    t_sum = sum(p["us"] for p in per) / 1e6
-    achieved = bytes_step / t_sum / 1e9
+            prof = json.load(f)
+        same = all(prof["per_size"].get(str(q["N"]), {}).get("radices") == q["radices"]
+                   for q in per)
+        if same:
+            traffic = round(prof["step_traffic_bytes"])
+            traffic_src = ("stored rocprofv3 PMC summary of this sweep's plans "
+                           "(profiles/traffic_pow2_sweep.json), not measured in this run")
     res = {
         "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t * 1e3, 4),
@@ -170,20 +224,65 @@ def bench_sweep(args, torch, lf):
                                "uniform[-0.5,0.5) re/im; one step = all 18 sizes",
                    "sizes": len(sizes), "elements_per_size": total,
                    "planner": "measure" if args.measure else "estimate"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "the FFT kernels of one sweep step (tile_kernel, fixed_kernel "
-                               "and stage_reg_kernel launches, 22 per step): algorithmic 32N "
-                               "bytes per transform over "
-                               "the summed per-size HIP-event time; traffic = HBM bytes per step "
-                               "from rocprofv3 FETCH_SIZE + WRITE_SIZE, "
-                               "profiles/traffic_pow2_sweep.json"},
+        "roofline": {"bound": "hbm", "achieved": round(value, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(value / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "kernel": "every FFT kernel of one sweep step: algorithmic 32N bytes per "
+                               "transform over the timed step (the value's own clock)"},
         "per_size": per,
+        "per_size_sum_us": round(sum(q["us"] for q in per), 2),
     }
+    if len(sizes) == 18:
+        res["dominant_kernel"] = dominant_kernel(torch, plans[65536], views[65536], args)
+    if not args.no_extras and len(sizes) == 18:
+        res["ragged"] = ragged_summary(args, torch, lf, value)
+        res["autograd"] = autograd_summary(args, torch, lf)
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline_sweep(sizes, {N: plans[N].radices for N in sizes},
                                                  args.cpu_seconds)
     return res
+
+
+def dominant_kernel(torch, plan, view, args):
+    """The kernel with the largest share of a sweep step: the register stage kernel of 256
+    points at 16 sub-problems per workgroup (both stages of 2^16, the second of 2^15, the
+    first of 2^17). Timed live through the 2^16 plan, whose call is two launches of it; each
+    launch carries half of the batch's algorithmic bytes (32 N per transform)."""
+    xi, yi = view
+    for _ in range(max(1, args.warmup)):
+        plan.call(xi, yi)
+    reps = max(5, args.steps)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        plan.call(xi, yi)
+    e1.record()
+    torch.cuda.synchronize()
+    launch = e0.elapsed_time(e1) / 1e3 / reps / 2
+    alg = 32.0 * (1 << 24) / 2
+    return {"kernel": "mfft::stage_reg_kernel<true, 256, 8, 16, 8, 8, 4>", "bound": "hbm",
+            "launch_us": round(launch * 1e6, 2), "algorithmic_bytes_per_launch": alg,
+            "achieved": round(alg / launch / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(alg / launch / 1e9 / HBM_PEAK_GBS, 4),
+            "note": "a stage of a two-stage transform: each launch reads and writes the whole "
+                    "batch once, so the HBM bytes per launch are twice the algorithmic share"}
+
+
+def ragged_summary(args, torch, lf, sweep_gbs):
+    r = bench_ragged(args, torch, lf)
+    return {"GB/s": r["value"], "frac": r["roofline"]["frac"], "ms": r["ms_per_step"],
+            "launches": 2, "transforms": 10000, "ragged_vs_uniform": r["ragged_vs_uniform"],
+            "uniform_same_sizes_GB/s": r["uniform_same_sizes"]["GB/s"],
+            "ragged_vs_sweep": round(r["value"] / sweep_gbs, 4),
+            "ragged_vs_per_size_launches": r["ragged_vs_per_size"]}
+
+
+def autograd_summary(args, torch, lf):
+    r = bench_autograd(args, torch, lf)
+    c = r["config"]
+    return {"fwd_bwd_over_fwd": r["value"], "graph_fwd_ms": c["graph_fwd_ms"],
+            "graph_fwd_bwd_ms": c["graph_fwd_bwd_ms"], "eager_ratio": c["eager_ratio"],
+            "shape": [512, 4096]}
 
 
 def bench_single(args, torch, lf):
@@ -246,9 +345,10 @@ def bench_mixed(args, torch, lf):
 
 def bench_ragged(args, torch, lf):
     """Config 4: 10,000 transforms of log-uniform 13-smooth sizes in [8, 65536], packed back
-    to back, one persistent launch, captured in a HIP graph. Compared with per-size plain
-    launches: the same transforms regrouped by size (contiguous per size) and run as one
-    batched uniform launch per distinct size."""
+    to back, run as two persistent launches from size-sorted queues (stage A and single-stage
+    tiles, then stage B), captured in one HIP graph. Compared with per-size plain launches
+    (the same transforms regrouped by size, one batched uniform launch per distinct size) and
+    with each distinct size's own uniform throughput."""
     sizes = lf.workloads.ragged_sizes()
     total = sum(sizes)
     x = uniform_complex(torch, (total,), 44)
@@ -305,8 +405,8 @@ def bench_ragged(args, torch, lf):
             "data": "synthetic",
             "config": {"workload": "ragged batch: 10,000 complex FP64 forward transforms, sizes "
                                    "log-uniform over the 13-smooth integers in [8, 65536] (PCG64 "
-                                   "seed 20261203), packed back to back, one persistent launch in "
-                                   "a HIP graph", "sum_N": total, "distinct_sizes": len(cnt),
+                                   "seed 20261203), packed back to back, two persistent launches "
+                                   "in one HIP graph", "sum_N": total, "distinct_sizes": len(cnt),
                        "plan_seconds": round(plan_s, 2)},
             "per_size_plain": {"GB/s": round(b / t_ps / 1e9, 2), "ms": round(t_ps * 1e3, 4),
                                "launches": len(cnt)},
@@ -491,7 +591,9 @@ def main():
                              "2d"])
     ap.add_argument("--sizes", type=int, nargs="*", help="restrict the sweep to these sizes")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="sweep only: leave out the ragged and autograd parts of the metric")
+    ap.add_argument("--cpu-seconds", type=float, default=30.0)
     ap.add_argument("--measure", action="store_true",
                     help="plan with MFFT_MEASURE (time candidate schedules before the run)")
     args = ap.parse_args()

@@ -37,6 +37,11 @@ LIB = os.path.join(BUILD, "liboracle.so")
 _lock = threading.Lock()
 _lib = None
 
+# -O3 for the timed CPU baseline; x86-64-v3 (AVX2) rather than -march=native because the
+# library is built once and also runs on other hosts; no contraction and no fast-math, so
+# every butterfly rounds exactly as written
+CFLAGS = ["-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fno-fast-math"]
+
 
 def build(force: bool = False) -> str:
     """Generate the butterflies and compile liboracle.so (gcc, -ffp-contract=off)."""
@@ -56,7 +61,7 @@ def build(force: bool = False) -> str:
             return LIB
     tmp = LIB + ".tmp"
     cmd = [
-        "gcc", "-O2", "-march=x86-64-v3", "-ffp-contract=off", "-fno-fast-math",
+        "gcc", *CFLAGS,
         "-std=gnu11", "-shared", "-fPIC", "-o", tmp, *srcs, "-lquadmath", "-lm",
     ]
     subprocess.run(cmd, check=True)
@@ -77,6 +82,8 @@ def lib():
                 ctypes.c_int64, dp, dp, dp,
             ]
             L.oracle_ref_transform.restype = ctypes.c_int
+            L.oracle_ref_repeat.argtypes = L.oracle_ref_transform.argtypes
+            L.oracle_ref_repeat.restype = ctypes.c_int
             L.oracle_butterfly.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
             L.oracle_butterfly.restype = ctypes.c_int
             L.oracle_mp_dft.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, dp, dp, dp]
@@ -276,6 +283,16 @@ class RefPlan:
         self.tw = tw if tw.size else np.zeros(1)
         self.rsa = (ctypes.c_int * len(self.rs))(*self.rs)
         self.work = np.empty(4 * self.N)
+
+    def repeat(self, x: np.ndarray, out: np.ndarray, reps: int) -> None:
+        """The one-transform plan `reps` times on the same x (shape (N,)) into out."""
+        rc = lib().oracle_ref_repeat(
+            1 if self.fwd else 0, self.N, len(self.rs), self.rsa, _dp(self.tw), int(reps),
+            _dp(x.reshape(-1).view(np.float64)), _dp(out.reshape(-1).view(np.float64)),
+            _dp(self.work),
+        )
+        if rc != 0:
+            raise ValueError("oracle cannot run this plan")
 
     def execute(self, x: np.ndarray, out: np.ndarray) -> None:
         """x, out: complex128 arrays of shape (howmany, N), C-contiguous."""

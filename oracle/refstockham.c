@@ -74,6 +74,18 @@ int oracle_ref_transform(int fwd, int64_t N, int npass, const int *rs, const dou
      return 0;
 }
 
+/* The CPU-baseline timing loop: the same one-transform plan applied `reps` times to the
+ * same input and output (as libbench2's doit loops the reference's lfa_call,
+ * bench/backends/bench-leanfftw.c:281-291), so that the timing holds no per-call overhead
+ * of the caller. */
+int oracle_ref_repeat(int fwd, int64_t N, int npass, const int *rs, const double *tw,
+                      int64_t reps, const double *in, double *out, double *work)
+{
+     for (int64_t r = 0; r < reps; ++r)
+          if (oracle_ref_transform(fwd, N, npass, rs, tw, 1, in, out, work) != 0) return -1;
+     return 0;
+}
+
 /* One butterfly, for the tests: s has n_inputs(R, tw) doubles, y gets 2R. */
 int oracle_butterfly(int fwd, int R, int tw, const double *s, double *y)
 {
