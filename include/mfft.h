@@ -205,6 +205,19 @@ double mfft_tan_2pi(uint64_t num, uint64_t den);
  * real parts then ns imaginary parts (2 (R-1) ns doubles). */
 int mfft_pass_twiddles(int fwd, int64_t N, int R, int64_t ns, double *out);
 
+/* The reference's operation count of a Stockham schedule (stockhamCost,
+ * LeanFFTW/Gen/OpCount.lean:55-69): per pass t, N / R_t butterflies (twiddled from the second
+ * pass on), each costing weights[0] per addition, [1] per multiplication, [2] per fused
+ * multiply-add of its generated code, [3] per scalar load (2R inputs, 2(R-1) twiddle parts)
+ * and [4] per scalar store (2R). -1 for an invalid schedule. */
+int64_t mfft_stockham_cost(int fwd, int64_t n, const int *radices, int nradix,
+                           const int64_t *weights);
+
+/* The measuring planner's model of a schedule's GPU time, nanoseconds per transform: HBM
+ * passes (kernel stages, split > 0 for two), LDS passes, arithmetic and twiddle traffic; it
+ * orders and prunes the race (LeanFFTW/Asm/Planner.lean:185-205), the timings decide. */
+double mfft_schedule_cost(int fwd, int64_t n, const int *radices, int nradix, int split);
+
 /* Accepted for source compatibility with lfa_set_threads (Planner.lean:305-306); the GPU
  * grid replaces the CPU worker pool, so this only records the value. */
 int mfft_set_threads(size_t n);

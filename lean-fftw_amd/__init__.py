@@ -86,6 +86,10 @@ def lib():
                 "mfft_cos_sin_2pi": (None, [ctypes.c_uint64, ctypes.c_uint64, dp, dp]),
                 "mfft_tan_2pi": (ctypes.c_double, [ctypes.c_uint64, ctypes.c_uint64]),
                 "mfft_pass_twiddles": (ctypes.c_int, [ctypes.c_int, i64, ctypes.c_int, i64, dp]),
+                "mfft_stockham_cost": (i64, [ctypes.c_int, i64, ctypes.POINTER(ctypes.c_int),
+                                             ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]),
+                "mfft_schedule_cost": (ctypes.c_double, [ctypes.c_int, i64, ctypes.POINTER(ctypes.c_int),
+                                                         ctypes.c_int, ctypes.c_int]),
                 "mfft_set_threads": (ctypes.c_int, [ctypes.c_size_t]),
                 "mfft_set_wisdom_path": (ctypes.c_int, [ctypes.c_char_p]),
                 "mfft_error_string": (ctypes.c_char_p, [ctypes.c_int]),
@@ -167,6 +171,26 @@ def schedule(n: int):
     if k < 0:
         raise MfftError(f"unsupported size {n}")
     return list(buf[:k]), split.value
+
+
+def stockham_cost(fwd: bool, n: int, radices, weights) -> int:
+    """The reference's operation count of a schedule (stockhamCost) with weights
+    (add, mul, fma, load, store)."""
+    rs = (ctypes.c_int * len(radices))(*radices)
+    w = (ctypes.c_int64 * 5)(*weights)
+    v = lib().mfft_stockham_cost(1 if fwd else 0, n, rs, len(radices), w)
+    if v < 0:
+        raise MfftError(f"invalid schedule {radices} for {n}")
+    return int(v)
+
+
+def schedule_cost(fwd: bool, n: int, radices, split: int = 0) -> float:
+    """The measuring planner's modelled GPU time of a schedule (ns per transform)."""
+    rs = (ctypes.c_int * len(radices))(*radices)
+    v = lib().mfft_schedule_cost(1 if fwd else 0, n, rs, len(radices), split)
+    if v < 0:
+        raise MfftError(f"invalid schedule {radices} / {split} for {n}")
+    return float(v)
 
 
 def set_threads(n: int) -> None:

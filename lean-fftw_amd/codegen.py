@@ -421,6 +421,25 @@ def _lit(v: float) -> str:
     return "0.0" if v == 0.0 else v.hex()
 
 
+def op_kinds(fwd: bool, R: int, tw: bool) -> tuple:
+    """(additions, multiplications, fused multiply-adds) reachable from the outputs: add, sub
+    and neg count as additions, fma, fms and fnma as fused operations (the kinds of the
+    reference's Weights, LeanFFTW/Gen/OpCount.lean:23-49)."""
+    d, outs = butterfly_dag(fwd, R, tw)
+    seen, stack = set(), list(outs)
+    n = {"add": 0, "mul": 0, "fma": 0}
+    kind = {"add": "add", "sub": "add", "neg": "add", "mul": "mul", "fma": "fma", "fms": "fma",
+            "fnma": "fma"}
+    while stack:
+        i = stack.pop()
+        if i in seen or d.op[i][0] in ("in", "cst", "zero"):
+            continue
+        seen.add(i)
+        n[kind[d.op[i][0]]] += 1
+        stack.extend(d.op[i][1:])
+    return n["add"], n["mul"], n["fma"]
+
+
 def emit_butterfly(fwd: bool, R: int, tw: bool) -> str:
     d, outs = butterfly_dag(fwd, R, tw)
     # reachable nodes in id order (ids are created after their operands: topological)
@@ -484,6 +503,14 @@ def emit_all(radices=RADICES) -> str:
     ops = [op_count(True, R, True) if R in radices else 0 for R in range(17)]
     out.append("// arithmetic operations of the twiddled forward butterfly of each radix")
     out.append("constexpr int kBflyOps[17] = {%s};" % ", ".join(map(str, ops)))
+    out.append("// (additions, multiplications, fused multiply-adds) of butterfly [forward][twiddled][R]")
+    rows = []
+    for fwd in (False, True):
+        for tw in (False, True):
+            cells = ["{%d, %d, %d}" % (op_kinds(fwd, R, tw) if R in radices else (0, 0, 0))
+                     for R in range(17)]
+            rows.append("{" + ", ".join(cells) + "}")
+    out.append("constexpr int kBflyOpKinds[2][2][17][3] = {{%s, %s}, {%s, %s}};" % tuple(rows))
     out.append("}  // namespace mfft")
     return "\n".join(out) + "\n"
 

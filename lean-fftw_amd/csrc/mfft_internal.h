@@ -150,6 +150,9 @@ hipError_t launch_real_part(const double2 *work, double *out, int64_t n, int64_t
 // Fixed-schedule tile kernels (mfft_tile.hip): the kernel for a stage whose shape, radices,
 // lane orders and tile copies match one (1 + its index, 0 if none), and its launch.
 int tile_lookup(const StageArgs &a);
+bool tile_fits(const StageArgs &a, int tile);
+int tile_candidates(const StageArgs &a, int *tiles, int max);
+bool tile_shape(int tile, int *G, int *NT);
 int xcd_remap(const StageArgs &a, int G);
 hipError_t launch_tile(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,
                        double2 *out, const double2 *tw, hipStream_t stream);

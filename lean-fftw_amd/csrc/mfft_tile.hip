@@ -148,6 +148,42 @@ int tile_lookup(const StageArgs &a) {
   }
 }
 
+// Whether tile kernel `tile` (1 + index) fits the stage: its shape, radices, lane orders and
+// tile copies, and at least one full tile of sub-problems.
+bool tile_fits(const StageArgs &a, int tile) {
+  const TileSchedule *t = tile_at(tile - 1);
+  if (!t || a.reg == 2) return false;
+  if (t->L != a.L || t->npass != a.npass || t->gfi != (a.gfast_in ? 1 : 0) ||
+      t->gfo != (a.gfast_out ? 1 : 0) || t->io != a.stage_io || a.nsub < t->G)
+    return false;
+  for (int p = 0; p < a.npass; ++p)
+    if (t->rs[p] != a.pass[p].R) return false;
+  return true;
+}
+
+// Every tile kernel that fits the stage (1 + index each, table order), at most `max`; returns
+// how many fit. The autotuner races them.
+int tile_candidates(const StageArgs &a, int *tiles, int max) {
+  int n = 0;
+  for (int i = 0;; ++i) {
+    if (!tile_at(i)) return n;
+    if (tile_fits(a, i + 1)) {
+      if (n < max) tiles[n] = i + 1;
+      ++n;
+    }
+  }
+}
+
+// The tile (sub-problems, threads) of tile kernel `tile` (1 + index), the identity wisdom
+// records.
+bool tile_shape(int tile, int *G, int *NT) {
+  const TileSchedule *t = tile_at(tile - 1);
+  if (!t) return false;
+  *G = t->G;
+  *NT = t->NT;
+  return true;
+}
+
 hipError_t launch_tile(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,
                        double2 *out, const double2 *tw, hipStream_t stream) {
   const TileSchedule *tp = tile_at(a.tile - 1);

@@ -8,6 +8,9 @@
 #include "mfft_internal.h"
 
 #include <cstring>
+#include <list>
+#include <memory>
+#include <mutex>
 #include <numeric>
 #include <thread>
 
@@ -317,7 +320,27 @@ double tan_2pi(uint64_t num, uint64_t den) {
 
 // Trig.lean:131-134 -- every reduced angle is a multiple of g = gcd(8, n): evaluate the
 // n/g + 1 of them once, then map each m < n through the octant symmetries.
+namespace {
+
+// The last few tables (sizes up to 2^20), so that planning many candidate schedules of one
+// size (the measuring planner) computes its table once.
+std::mutex g_table_mu;
+std::list<std::pair<uint64_t, std::shared_ptr<const std::vector<double>>>> g_tables;
+constexpr size_t kTableCache = 4;
+
+}  // namespace
+
 void cos_sin_2pi_table(uint64_t n, std::vector<double> &cs) {
+  const bool cacheable = n <= (uint64_t(1) << 20);
+  if (cacheable) {
+    std::lock_guard<std::mutex> lk(g_table_mu);
+    for (auto it = g_tables.begin(); it != g_tables.end(); ++it)
+      if (it->first == n) {
+        cs = *it->second;
+        g_tables.splice(g_tables.begin(), g_tables, it);  // most recent first
+        return;
+      }
+  }
   cs.assign(2 * n, 0.0);
   const uint64_t g = std::gcd<uint64_t>(8, n);
   const uint64_t count = n / g + 1;
@@ -336,6 +359,12 @@ void cos_sin_2pi_table(uint64_t n, std::vector<double> &cs) {
     const CS r = octant(memo[(oct % 2 == 0 ? rem : n - rem) / g], oct);
     cs[2 * m] = r.c;
     cs[2 * m + 1] = r.s;
+  }
+  if (cacheable) {
+    auto t = std::make_shared<const std::vector<double>>(cs);
+    std::lock_guard<std::mutex> lk(g_table_mu);
+    g_tables.emplace_front(n, std::move(t));
+    if (g_tables.size() > kTableCache) g_tables.pop_back();
   }
 }
 

@@ -256,3 +256,43 @@ def test_ragged_queue_covers_every_tile_once(lf):
 
 def shapes_n_stages(shapes, n):
     return [s[1] for s in shapes if s[0] == n][0]
+
+
+PLANS_COST = [(1024, [16, 8, 8]), (1024, [8, 4, 4, 8]), (1000, [10, 10, 10]), (2100, [15, 14, 10]),
+              (1001, [13, 11, 7]), (75600, [10, 9, 3, 10, 7, 4]), (165375, [5, 5, 5, 3, 9, 7, 7]),
+              (262144, [8, 8, 8, 8, 8, 8]), (6, [6]), (12, [12])]
+
+
+@pytest.mark.parametrize("fwd", [True, False])
+def test_stockham_cost_against_oracle_counts(lf, fwd):
+    """mfft_stockham_cost (the reference's stockhamCost, LeanFFTW/Gen/OpCount.lean:55-69) from
+    the product's generated butterflies equals the same sum over the oracle's butterfly DAGs
+    (whose counts are pinned to the reference's published ones), for several weightings; the
+    memory weights alone give the closed form 4R loads / stores per untwiddled and 6R - 2 per
+    twiddled butterfly."""
+    from oracle.symgen import kernel_counts
+
+    for n, rs in PLANS_COST:
+        for W in ((1, 1, 1, 0, 0), (1, 1, 2, 0, 0), (3, 5, 7, 11, 13)):
+            want = 0
+            for t, R in enumerate(rs):
+                a, m, f = kernel_counts(fwd, R, t >= 1)
+                loads = 2 * R + (2 * (R - 1) if t >= 1 else 0)
+                want += (n // R) * (W[0] * a + W[1] * m + W[2] * f + W[3] * loads + W[4] * 2 * R)
+            assert lf.stockham_cost(fwd, n, rs, W) == want, (n, rs, W)
+        mem = sum((n // R) * (4 * R if t == 0 else 6 * R - 2) for t, R in enumerate(rs))
+        assert lf.stockham_cost(fwd, n, rs, (0, 0, 0, 1, 1)) == mem
+    with pytest.raises(lf.MfftError):
+        lf.stockham_cost(True, 1024, [16, 8, 4], (1, 1, 1, 0, 0))
+
+
+def test_schedule_cost_orders_stages_and_passes(lf):
+    """The tuner's GPU model: a second kernel stage (another HBM round trip) costs more than
+    any single-stage schedule of the size; a further LDS pass costs more than fewer passes of
+    the same arithmetic class."""
+    one = lf.schedule_cost(True, 4096, [8, 8, 8, 8], 0)
+    two = lf.schedule_cost(True, 4096, [8, 8, 8, 8], 2)
+    assert two > 1.4 * one
+    assert lf.schedule_cost(True, 4096, [4, 4, 4, 4, 4, 4], 0) > lf.schedule_cost(True, 4096, [16, 16, 16], 0)
+    # HBM dominates: 32 N bytes at 6.5 TB/s per stage
+    assert 32 * 4096 / 6.5e3 < one < 2 * 32 * 4096 / 6.5e3

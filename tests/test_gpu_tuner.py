@@ -92,3 +92,70 @@ def test_measured_strided_plan(lf, oracle, tmp_path):
         assert bits_equal(got, oracle.ref_transform(True, p.radices, x).reshape(-1))
     finally:
         lf.set_wisdom_path(None)
+
+
+TUNE_SIZES = [2 ** k for k in range(1, 19)] + [6, 60, 360, 1000, 2100, 10368, 75600, 165375]
+
+
+def _time(plan, x, y, reps=3, k=10):
+    """us per call: the fastest of `reps` timings of `k` back-to-back calls."""
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    best = float("inf")
+    for _ in range(reps):
+        e0.record()
+        for _ in range(k):
+            plan.call(x, y)
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e3 / k)
+    return best
+
+
+@pytest.mark.parametrize("n", TUNE_SIZES)
+def test_measure_not_slower_than_estimate(lf, tmp_path, n):
+    """MEASURE races the planner's own schedule (always a candidate) against the cost-model
+    schedules and every tile kernel of their stages; on a 64 MB batch the plan it keeps is no
+    slower than ESTIMATE's (both timed interleaved, min of 5 rounds of 3 timings of 10
+    back-to-back calls; 5% + 1 us of noise allowed)."""
+    lf.set_wisdom_path(str(tmp_path / "wisdom.jsonl"))
+    try:
+        hm = max(1, (1 << 22) // n)
+        pe = lf.Plan(n, -1, hm)
+        pm = lf.Plan(n, -1, hm, flags=lf.MEASURE)
+        g = torch.Generator(device=DEV).manual_seed(n)
+        x = torch.complex(torch.rand(hm, n, dtype=torch.float64, device=DEV, generator=g),
+                          torch.rand(hm, n, dtype=torch.float64, device=DEV, generator=g))
+        y = torch.empty_like(x)
+        for p in (pe, pm):
+            p.call(x, y)
+        te = tm = float("inf")
+        for _ in range(5):
+            te = min(te, _time(pe, x, y))
+            tm = min(tm, _time(pm, x, y))
+        assert tm <= 1.05 * te + 1.0, (n, pe.radices, pe.stages, pm.radices, pm.stages, te, tm)
+        line = json.loads((tmp_path / "wisdom.jsonl").read_text().splitlines()[-1])
+        assert line["rs"] == pm.radices and len(line["tg"]) == 2 and len(line["tn"]) == 2
+    finally:
+        lf.set_wisdom_path(None)
+
+
+def test_wisdom_tile_choice_is_obeyed(lf, oracle, tmp_path):
+    """A wisdom line naming the kernel of each stage: 'no tile' runs the register kernel, a
+    tile that does not fit falls back to the planner's own kernel; either way the result is
+    the oracle's for that radix sequence."""
+    path = tmp_path / "wisdom.jsonl"
+    lf.set_wisdom_path(str(path))
+    try:
+        n, hm = 1024, 64
+        lf.Plan(n, -1, hm, flags=lf.MEASURE)
+        key = json.loads(path.read_text().splitlines()[-1])["key"]
+        x = oracle.lcg_input(n, seed=12, howmany=hm)
+        for tg, tn in (([0, -1], [0, -1]), ([5, -1], [999, -1]), ([2, -1], [256, -1])):
+            with open(path, "a") as f:
+                f.write(json.dumps({"key": key, "rs": [8, 8, 8, 2], "split": 0, "tg": tg,
+                                    "tn": tn, "us": 1.0}) + "\n")
+            p = lf.Plan(n, -1, hm, flags=lf.MEASURE)
+            assert p.radices == [8, 8, 8, 2]
+            assert bits_equal(run_plan(p, x), oracle.ref_transform(True, [8, 8, 8, 2], x))
+    finally:
+        lf.set_wisdom_path(None)
