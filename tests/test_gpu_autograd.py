@@ -150,8 +150,10 @@ def test_partials_every_kernel_shape(lf, oracle, N, B):
 
 
 @pytest.mark.skip(reason="host segfault inside torch's CUDAGraph.capture_end when the autograd "
-                         "engine's backward runs under stream capture; cause not identified, "
-                         "case kept off the GPU (see the design notes)")
+                         "engine's backward ran under stream capture; two hazards on that path "
+                         "were fixed (launch status from the thread's last error, plan frees "
+                         "during a capture) but neither is shown to be the cause, so the case "
+                         "stays off the GPU")
 def test_graph_captured_fwd_bwd_replays(lf):
     """The fused loss and its backward captured in a HIP graph: every replay, with eager work
     in between, matches the eager result bit for bit."""
