@@ -15,7 +15,9 @@
  * Conventions (docs/CONTRACT.md:9-17 of the reference): forward is sign -1,
  * Y[k] = sum_j x[j] exp(-2 pi i j k / N); backward is sign +1; nothing is normalised.
  * Data is interleaved complex binary64 (re, im). Sizes: 2 <= N <= 2^24 with prime factors
- * <= 13, as the reference's fast engine (bench-leanfftw.c:253-268).
+ * <= 13, as the reference's fast engine (bench-leanfftw.c:253-268): one kernel stage up to a
+ * few thousand points, two up to 2048 x 2048 where N splits so, three otherwise (a 2^24-point
+ * transform is 256 MiB). Column layouts (|dist| < |stride|) take one or two stages only.
  *
  * Differences from the reference, by design:
  *   - Device pointers: the execute calls take device (HBM) pointers and a HIP stream and
@@ -155,6 +157,13 @@ int mfft_plan_schedule(int64_t n, int *radices, int max, int *split);
  * <= 13) and, if stage_split > 0, the first kernel stage runs passes [0, stage_split). */
 mfft_plan *mfft_plan_with_radices(int64_t n, int sign, int64_t howmany, const int *radices,
                                   int nradix, int stage_split);
+
+/* Force a radix sequence and its cut into kernel stages (tests and tuning): `passes` lists the
+ * number of passes of each of the nstage (1..3) stages, summing to nradix; contiguous batches.
+ * NULL if a stage's sub-problem length exceeds the LDS (2048 points) or the schedule is
+ * invalid. */
+mfft_plan *mfft_plan_with_stages(int64_t n, int sign, int64_t howmany, const int *radices,
+                                 int nradix, const int *passes, int nstage);
 
 /* Ragged batch: `count` transforms of sizes sizes[i], packed back to back (transform i
  * starts at element sum_{j<i} sizes[j]), one direction, executed as two persistent launches

@@ -61,6 +61,8 @@ def lib():
                 "mfft_plan_kind": (ctypes.c_int, [vp]),
                 "mfft_plan_with_radices": (vp, [i64, ctypes.c_int, i64, ctypes.POINTER(ctypes.c_int),
                                                 ctypes.c_int, ctypes.c_int]),
+                "mfft_plan_with_stages": (vp, [i64, ctypes.c_int, i64, ctypes.POINTER(ctypes.c_int),
+                                               ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
                 "mfft_execute": (ctypes.c_int, [vp, vp, vp, vp]),
                 "mfft_execute_host": (ctypes.c_int, [vp, dp, dp]),
                 "mfft_execute_static": (ctypes.c_int, [vp, vp, vp, vp]),
@@ -210,12 +212,13 @@ class Plan:
     """A batched 1-D complex FP64 transform on the current device.
 
     Plan(n, sign=FORWARD, howmany=1, radices=None, split=0, istride=1, idist=None,
-    ostride=1, odist=None, flags=ESTIMATE). `radices` forces the Stockham radix sequence
-    (the reference's `rs`); `split` the number of passes of the first kernel stage.
+    ostride=1, odist=None, flags=ESTIMATE, passes=None). `radices` forces the Stockham radix
+    sequence (the reference's `rs`); `split` the number of passes of the first kernel stage, or
+    `passes` the passes of each of up to three stages.
     """
 
     def __init__(self, n, sign=FORWARD, howmany=1, radices=None, split=0, istride=1,
-                 idist=None, ostride=1, odist=None, flags=ESTIMATE, _handle=None):
+                 idist=None, ostride=1, odist=None, flags=ESTIMATE, _handle=None, passes=None):
         L = lib()
         self.n, self.sign, self.howmany = int(n), int(sign), int(howmany)
         self.istride, self.ostride = int(istride), int(ostride)
@@ -227,7 +230,13 @@ class Plan:
             if (self.istride, self.ostride, self.idist, self.odist) != (1, 1, self.n, self.n):
                 raise ValueError("forced radices are for contiguous batches")
             arr = (ctypes.c_int * len(radices))(*radices)
-            h = L.mfft_plan_with_radices(self.n, self.sign, self.howmany, arr, len(radices), int(split))
+            if passes is not None:
+                pa = (ctypes.c_int * len(passes))(*passes)
+                h = L.mfft_plan_with_stages(self.n, self.sign, self.howmany, arr, len(radices), pa,
+                                            len(passes))
+            else:
+                h = L.mfft_plan_with_radices(self.n, self.sign, self.howmany, arr, len(radices),
+                                             int(split))
         else:
             h = L.mfft_plan_many(self.n, self.sign, self.howmany, self.istride, self.idist,
                                  self.ostride, self.odist, flags)

@@ -85,6 +85,9 @@ struct StageArgs {
   int64_t in_g, in_e, in_h, out_g, out_e, out_h;
   int64_t kb_u;
   int64_t kb_h;  // twiddle base of workgroup row h: kb = g * kb_u + h * kb_h
+  // grid.z (the stage kernel only; the middle stage of a three-stage plan, where grid.y walks a
+  // second sub-problem index): element offsets in_z, out_z per z
+  int64_t in_z, out_z;
   // real-data element maps fused into the first loads / last stores (register kernels only):
   // io_in 0 complex, 1 real doubles, 2 Hermitian half; io_out 0 complex, 1 real parts,
   // 2 the first io_n / 2 + 1 elements; io_n the full transform length
@@ -180,9 +183,10 @@ int64_t workspace_elems(const ::mfft_plan *p);
 // The device a plan was made on.
 int plan_device(const ::mfft_plan *p, int *dev);
 
-// Launch one stage over `ntrans` transforms; returns a hipError_t.
+// Launch one stage over `ntrans` rows of workgroups (grid.y) and `nz` layers (grid.z, the stage
+// kernel only); returns a hipError_t.
 hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,
-                        double2 *out, const double2 *tw, hipStream_t stream);
+                        double2 *out, const double2 *tw, hipStream_t stream, int64_t nz = 1);
 
 inline uint32_t magic32(int64_t d) {
   return d <= 1 ? 0u : (uint32_t)(((uint64_t(1) << 32) + (uint64_t)d - 1) / (uint64_t)d);

@@ -17,11 +17,11 @@ __global__ __launch_bounds__(MAXT) void stage_kernel(const double2 *in, double2 
                                                      const StageArgs a) {
   extern __shared__ double2 lds[];  // G padded sub-problems
   const int g0 = blockIdx.x * a.G;
-  const int64_t h = blockIdx.y;
+  const int64_t h = blockIdx.y, z = blockIdx.z;
   const int gcount = (int)(a.nsub - g0 < a.G ? a.nsub - g0 : a.G);
-  HbmSide hin{const_cast<double2 *>(in) + h * a.in_h + g0 * a.in_g, nullptr, a.in_g, a.in_e,
-              a.pre ? a.pre + g0 * a.pre_g : nullptr, a.pre_g, a.pre_e, a.cached_in};
-  HbmSide hout{out + h * a.out_h + g0 * a.out_g, nullptr, a.out_g, a.out_e,
+  HbmSide hin{const_cast<double2 *>(in) + z * a.in_z + h * a.in_h + g0 * a.in_g, nullptr, a.in_g,
+              a.in_e, a.pre ? a.pre + g0 * a.pre_g : nullptr, a.pre_g, a.pre_e, a.cached_in};
+  HbmSide hout{out + z * a.out_z + h * a.out_h + g0 * a.out_g, nullptr, a.out_g, a.out_e,
                a.post ? a.post + g0 * a.post_g : nullptr, a.post_g, a.post_e, a.cached_out};
   // (the run-time-sided pass body of run_stage_compact measured the same here: MI355X, mixed
   // sizes on this kernel 2.91 / 2.92 TB/s)
@@ -61,8 +61,11 @@ const void *kernel_for(int fwd, int set, int ept, int sq, int big) {
 }
 
 hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,
-                        double2 *out, const double2 *tw, hipStream_t stream) {
-  if (a.nsub <= 0 || ntrans <= 0) return hipSuccess;
+                        double2 *out, const double2 *tw, hipStream_t stream, int64_t nz) {
+  if (a.nsub <= 0 || ntrans <= 0 || nz <= 0) return hipSuccess;
+  // a z extent only on the stage kernel (no tile, register, weights or partials)
+  if (nz > 1 && (a.tile > 0 || a.reg != 0 || a.sumsq || nz > 65535 || ntrans > 65535))
+    return hipErrorInvalidValue;
   // the real-data element maps exist in the register kernels (input maps at 16 elements per
   // thread: real input into a schedule opening with radix 16) and in the staged tile kernels
   const bool tile_io = a.tile > 0 && a.stage_io == 2 && !a.pre && !a.post && !a.sumsq;
@@ -76,7 +79,7 @@ hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const doub
   if (a.reg == 2 && !a.sumsq) return launch_reg_stage(fwd, a, ntrans, in, out, tw, stream);
   const int64_t nx = (a.nsub + a.G - 1) / a.G;
   const size_t lds = (size_t)stage_lds_elems(a.G, a.L, a.npass, a.stage_io) * sizeof(double2);
-  const dim3 grid((unsigned)nx, (unsigned)ntrans);
+  const dim3 grid((unsigned)nx, (unsigned)ntrans, (unsigned)nz);
   // variants: <direction, radix set, elements per thread, max threads, sum of squares>
   bool small = true, mid = true, pow2 = true;
   for (int p = 0; p < a.npass; ++p) {

@@ -225,6 +225,31 @@ bool choose_split(int64_t N, int64_t &P) {
   return P != 0;
 }
 
+// Split N into P1 * P2 * P3 for a three-stage plan (N whose two-way splits leave a factor
+// above the LDS): each within the LDS, as balanced as possible.
+bool choose_split3(int64_t N, int64_t &P1, int64_t &P2) {
+  P1 = P2 = 0;
+  double best = 1e300;
+  const double third = std::log((double)N) / 3.0;
+  for (int64_t a = 2; a <= mfft::LDS_ELEMS && a < N; ++a) {
+    if (N % a) continue;
+    for (int64_t b = 2; b <= mfft::LDS_ELEMS && a * b < N; ++b) {
+      if ((N / a) % b) continue;
+      const int64_t c = N / a / b;
+      if (c > mfft::LDS_ELEMS || c < 2) continue;
+      const double score = std::fabs(std::log((double)a) - third) +
+                           std::fabs(std::log((double)b) - third) +
+                           std::fabs(std::log((double)c) - third);
+      if (score < best) {
+        best = score;
+        P1 = a;
+        P2 = b;
+      }
+    }
+  }
+  return P1 != 0;
+}
+
 // Power-of-two sizes that run as one register-resident stage (mfft_reg.hip): from
 // MFFT_REG_MIN (default 64) to 16384 points; MFFT_REG=0 turns the kernel off.
 int64_t reg_min() {
@@ -424,8 +449,94 @@ bool build_columns(mfft_plan *p, int split, const std::vector<int64_t> &ns) {
   return true;
 }
 
-// Build the stage descriptors of a plan whose radices and split are set.
-bool build_stages(mfft_plan *p, int split) {
+// Three stages, N = P1 P2 P3 with passes [0, a), [a, b), [b, s) (ns_a = P1, ns_b = P1 P2): the
+// two-stage decomposition applied again to its second stage. A: columns c < N/P1 of length
+// P1, x[c + (N/P1) m] -> T[c P1 + k]. B: sub-problems (k1 < P1 along g, c2 < P3 along
+// grid.y, transform along grid.z) of length P2, T[k1 + P1 c2 + (N/P2) m] -> U[k1 + P1 P2 c2 +
+// P1 k], twiddle base k1 and multiplier P1. C: residue classes K < P1 P2 of length P3,
+// U[K + P1 P2 i] -> y[K + P1 P2 e], twiddle base K and multiplier P1 P2. Every pass is the
+// reference's, so the bits are refTransform's. B runs the stage kernel (the only one with a
+// z extent); A and C may take tile kernels.
+bool build_three(mfft_plan *p, int a, int b) {
+  const int64_t N = p->n;
+  const auto &rs = p->radices;
+  const int s = (int)rs.size();
+  if (a <= 0 || b <= a || b >= s || columns_layout(p)) return false;
+  std::vector<int64_t> ns(s + 1, 1);
+  for (int t = 0; t < s; ++t) ns[t + 1] = ns[t] * rs[t];
+  const int64_t P1 = ns[a], P2 = ns[b] / ns[a], P3 = N / ns[b], NB = ns[b];
+  if (P1 > mfft::LDS_ELEMS || P2 > mfft::LDS_ELEMS || P3 > mfft::LDS_ELEMS || a > mfft::MAX_PASSES ||
+      b - a > mfft::MAX_PASSES || s - b > mfft::MAX_PASSES)
+    return false;
+  const std::vector<int> ra(rs.begin(), rs.begin() + a), rb(rs.begin() + a, rs.begin() + b),
+      rc(rs.begin() + b, rs.end());
+  StageArgs A{}, B{}, C{};
+  A.L = (int)P1;
+  A.npass = a;
+  strided_tile(A, P1, N / P1, p->allow_big, ra);
+  for (int t = 0; t < a; ++t) fill_pass(A.pass[t], rs[t], ns[t], ns[t], 1, A.L, A.G);
+  A.nsub = N / P1;
+  A.in_g = p->istride;
+  A.in_e = p->istride * (N / P1);
+  A.in_h = p->idist;
+  A.out_g = P1;
+  A.out_e = 1;
+  A.out_h = N;
+  A.gfast_in = 1;
+  A.pre_g = 1;
+  A.pre_e = N / P1;
+  A.cached_out = 1;
+
+  B.L = (int)P2;
+  B.npass = b - a;
+  strided_tile(B, P2, P1, p->allow_big, rb);
+  for (int t = a; t < b; ++t) fill_pass(B.pass[t - a], rs[t], ns[t] / P1, ns[t], P1, B.L, B.G);
+  B.nsub = P1;
+  B.in_g = 1;
+  B.in_e = N / P2;
+  B.in_h = P1;      // grid.y: c2 < P3
+  B.in_z = N;       // grid.z: the transform
+  B.out_g = 1;
+  B.out_e = P1;
+  B.out_h = P1 * P2;
+  B.out_z = N;
+  B.kb_u = 1;
+  B.gfast_in = 1;
+  B.gfast_out = 1;
+  B.cached_in = 1;
+  B.cached_out = 1;
+
+  C.L = (int)P3;
+  C.npass = s - b;
+  strided_tile(C, P3, NB, p->allow_big, rc);
+  for (int t = b; t < s; ++t) fill_pass(C.pass[t - b], rs[t], ns[t] / NB, ns[t], NB, C.L, C.G);
+  C.nsub = NB;
+  C.in_g = 1;
+  C.in_e = NB;
+  C.in_h = N;
+  C.out_g = p->ostride;
+  C.out_e = p->ostride * NB;
+  C.out_h = p->odist;
+  C.kb_u = 1;
+  C.gfast_in = 1;
+  C.gfast_out = 1;
+  C.post_g = 1;
+  C.post_e = NB;
+  C.cached_in = 1;
+  p->stages = {A, B, C};
+  p->stage_passes = {a, b - a, s - b};
+  // equal chunks of at most chunk_bytes() over both intermediates
+  int64_t cmax = std::max<int64_t>(1, std::min<int64_t>(p->howmany, chunk_bytes() / (32 * N)));
+  cmax = std::min<int64_t>(cmax, 65535);  // grid.y of A and C, grid.z of B
+  const int64_t nchunk = (std::max<int64_t>(1, p->howmany) + cmax - 1) / cmax;
+  p->chunk_h = (std::max<int64_t>(1, p->howmany) + nchunk - 1) / nchunk;
+  return true;
+}
+
+// Build the stage descriptors of a plan whose radices and split(s) are set (split2 > 0: three
+// stages).
+bool build_stages(mfft_plan *p, int split, int split2 = 0) {
+  if (split2 > 0) return build_three(p, split, split2);
   const int64_t N = p->n;
   const auto &rs = p->radices;
   const int s = (int)rs.size();
@@ -568,8 +679,8 @@ bool build_stages(mfft_plan *p, int split) {
   return true;
 }
 
-mfft_plan *finish(mfft_plan *p, int split) {
-  if (!build_stages(p, split) || hipGetDevice(&p->device) != hipSuccess ||
+mfft_plan *finish(mfft_plan *p, int split, int split2 = 0) {
+  if (!build_stages(p, split, split2) || hipGetDevice(&p->device) != hipSuccess ||
       build_twiddles(p) != hipSuccess) {
     mfft_destroy_plan(p);
     return nullptr;
@@ -577,8 +688,10 @@ mfft_plan *finish(mfft_plan *p, int split) {
   // stages with a fixed-schedule tile kernel run it (two-stage column plans size their stages
   // per chunk and look theirs up at run time)
   for (auto &st : p->stages) st.tile = p->columns ? 0 : mfft::tile_lookup(st);
+  if (p->stages.size() == 3) p->stages[1].tile = 0;  // the middle stage needs grid.z
   if (p->stages.size() > 1) {
-    if (hipMalloc(&p->d_scratch, (size_t)p->chunk_h * p->n * sizeof(double2)) != hipSuccess) {
+    const size_t per = (p->stages.size() == 3 ? 2 : 1) * (size_t)p->n;  // intermediates
+    if (hipMalloc(&p->d_scratch, (size_t)p->chunk_h * per * sizeof(double2)) != hipSuccess) {
       mfft_destroy_plan(p);
       return nullptr;
     }
@@ -614,7 +727,8 @@ int work_for(const mfft_plan *p, const mfft::Workspace *ws, Work &w) {
     return MFFT_SUCCESS;
   }
   const int64_t hm = std::max<int64_t>(1, p->howmany);
-  const int64_t cmax = std::min<int64_t>({hm, ws->elems / p->n, 65535});
+  const int64_t per = p->stages.size() == 3 ? 2 * p->n : p->n;  // intermediates per transform
+  const int64_t cmax = std::min<int64_t>({hm, ws->elems / per, 65535});
   if (cmax < 1 || !ws->ptr) return MFFT_ERR_WORKSPACE;
   const int64_t nchunk = (hm + cmax - 1) / cmax;
   w = {ws->ptr, (hm + nchunk - 1) / nchunk};
@@ -646,6 +760,23 @@ int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
   }
   Work w{};
   if (const int rc = work_for(p, ws, w)) return rc;
+  if (p->stages.size() == 3) {  // x -> T -> U -> y, chunks of w.chunk transforms
+    const int64_t N = p->n, P3 = p->stages[2].L;
+    const int64_t nxc = (p->stages[2].nsub + p->stages[2].G - 1) / p->stages[2].G;
+    double2 *T = w.ptr, *U = w.ptr + w.chunk * N;
+    for (int64_t h0 = 0; h0 < p->howmany; h0 += w.chunk) {
+      const int64_t hc = std::min(w.chunk, p->howmany - h0);
+      StageArgs A = p->stages[0], B = p->stages[1], C = p->stages[2];
+      A.pre = pre;
+      C.post = post;
+      C.sumsq = sumsq ? sumsq + h0 * nxc : nullptr;
+      if (mfft::launch_stage(fwd, A, hc, in + h0 * p->idist, T, p->d_tw, st) != hipSuccess ||
+          mfft::launch_stage(fwd, B, P3, T, U, p->d_tw, st, hc) != hipSuccess ||
+          mfft::launch_stage(fwd, C, hc, U, out + h0 * p->odist, p->d_tw, st) != hipSuccess)
+        return MFFT_ERR_HIP;
+    }
+    return MFFT_SUCCESS;
+  }
   if (p->columns) {
     if (pre || post || sumsq) return MFFT_ERR_UNSUPPORTED;
     const int64_t P = p->stages[0].L, Q = p->stages[1].L;
@@ -1088,8 +1219,8 @@ mfft_plan *build_cand(const mfft_plan *shape, const Cand &c, int64_t howmany) {
   return q;
 }
 
-// The planner's own schedule (ESTIMATE) of a plan whose shape is set: radices and split.
-bool estimate_schedule(const mfft_plan *p, std::vector<int> &rs, int &split);
+// The planner's own schedule (ESTIMATE) of a plan whose shape is set: radices and split(s).
+bool estimate_schedule(const mfft_plan *p, std::vector<int> &rs, int &split, int &split2);
 
 // Expand a schedule into one candidate per combination of its stages' kernels: every tile
 // kernel that fits the stage, and no tile; the planner's own combination is marked.
@@ -1172,7 +1303,10 @@ bool measure(const mfft_plan *shape, Cand &best) {
   std::vector<Cand> scheds;
   {
     Cand seed;
-    if (estimate_schedule(shape, seed.rs, seed.split)) {
+    int split2 = 0;
+    const bool est = estimate_schedule(shape, seed.rs, seed.split, split2);
+    if (split2 > 0) return false;  // three stages: the planner's own schedule, not raced
+    if (est) {
       seed.seed = true;
       scheds.push_back(seed);
     }
@@ -1261,13 +1395,13 @@ bool measure(const mfft_plan *shape, Cand &best) {
 namespace {
 
 // The planner's own schedule (ESTIMATE) of a plan whose shape is set: the hand-measured
-// radices of the register-resident sizes, else the cost-model radices of one stage or of the
-// most balanced split.
-bool estimate_schedule(const mfft_plan *p, std::vector<int> &rs, int &split) {
+// radices of the register-resident sizes, else the cost-model radices of one stage, of the
+// most balanced split into two, or (no two-way split fits the LDS) of three.
+bool estimate_schedule(const mfft_plan *p, std::vector<int> &rs, int &split, int &split2) {
   const int64_t n = p->n;
   const bool pow2 = (n & (n - 1)) == 0;
   const bool contig = p->istride == 1 && p->ostride == 1;
-  split = 0;
+  split = split2 = 0;
   // columns too long for a tile of 4 (64-byte rows) run as two stages of wider rows rather
   // than one stage of narrower rows (MI355X, 2-D columns: 4096 points two stages 274-304 us
   // against 868), and so do power-of-two columns too long for a tile of 8, whose stages have
@@ -1281,7 +1415,18 @@ bool estimate_schedule(const mfft_plan *p, std::vector<int> &rs, int &split) {
     return true;
   }
   int64_t P;
-  if (!choose_split(n, P)) return false;
+  if (!choose_split(n, P)) {
+    int64_t P1, P2;
+    if (!p->allow_big || columns_layout(p) || !choose_split3(n, P1, P2)) return false;
+    const std::vector<int> a = choose_radices(P1), b = choose_radices(P2),
+                           c = choose_radices(n / P1 / P2);
+    rs = a;
+    rs.insert(rs.end(), b.begin(), b.end());
+    rs.insert(rs.end(), c.begin(), c.end());
+    split = (int)a.size();
+    split2 = (int)(a.size() + b.size());
+    return true;
+  }
   std::vector<int> a = choose_radices(P, !p->allow_big), b = choose_radices(n / P, !p->allow_big);
   rs = a;
   rs.insert(rs.end(), b.begin(), b.end());
@@ -1317,12 +1462,12 @@ mfft_plan *plan_many_internal(int64_t n, int sign, int64_t howmany, int64_t istr
       }
     }
   }
-  int split = 0;
-  if (!estimate_schedule(p, p->radices, split)) {
+  int split = 0, split2 = 0;
+  if (!estimate_schedule(p, p->radices, split, split2)) {
     delete p;
     return nullptr;
   }
-  return finish(p, split);
+  return finish(p, split, split2);
 }
 
 int mfft::execute_in(const mfft_plan *p, const void *in, void *out, const mfft::Workspace *ws,
@@ -1343,7 +1488,7 @@ int mfft::plan_device(const mfft_plan *p, int *dev) {
 
 int64_t mfft::workspace_elems(const mfft_plan *p) {
   if (!p) return 0;
-  if (p->kind == 0) return p->stages.size() > 1 ? p->n : 0;
+  if (p->kind == 0) return p->stages.size() == 3 ? 2 * p->n : p->stages.size() > 1 ? p->n : 0;
   int64_t e = 0;
   for (const mfft_plan *q : p->part) e = std::max(e, workspace_elems(q));
   return e;
@@ -1399,6 +1544,34 @@ mfft_plan *mfft_plan_with_radices(int64_t n, int sign, int64_t howmany, const in
   return finish(p, stage_split);
 }
 
+mfft_plan *mfft_plan_with_stages(int64_t n, int sign, int64_t howmany, const int *radices,
+                                 int nradix, const int *passes, int nstage) {
+  if (!passes || nstage < 1 || nstage > 3) return nullptr;
+  int sum = 0;
+  for (int i = 0; i < nstage; ++i) {
+    if (passes[i] < 1) return nullptr;
+    sum += passes[i];
+  }
+  if (sum != nradix) return nullptr;
+  if (nstage < 3) return mfft_plan_with_radices(n, sign, howmany, radices, nradix,
+                                                nstage == 2 ? passes[0] : 0);
+  if (!valid_problem(n, sign, howmany) || !radices) return nullptr;
+  int64_t prod = 1;
+  for (int i = 0; i < nradix; ++i) {
+    if (!radix_ok(radices[i])) return nullptr;
+    prod *= radices[i];
+  }
+  if (prod != n) return nullptr;
+  auto *p = new mfft_plan;
+  p->n = n;
+  p->sign = sign;
+  p->howmany = howmany;
+  p->istride = p->ostride = 1;
+  p->idist = p->odist = n;
+  p->radices.assign(radices, radices + nradix);
+  return finish(p, passes[0], passes[0] + passes[1]);
+}
+
 mfft_plan *mfft_plan_many(int64_t n, int sign, int64_t howmany, int64_t istride, int64_t idist,
                           int64_t ostride, int64_t odist, unsigned flags) {
   return plan_many_internal(n, sign, howmany, istride, idist, ostride, odist, flags, true);
@@ -1406,19 +1579,12 @@ mfft_plan *mfft_plan_many(int64_t n, int sign, int64_t howmany, int64_t istride,
 
 int mfft_plan_schedule(int64_t n, int *radices, int max, int *split) {
   if (!valid_problem(n, -1, 1)) return -1;
+  mfft_plan shape;
+  shape.n = n;
+  shape.idist = shape.odist = n;
   std::vector<int> rs;
-  int sp = 0;
-  const bool pow2 = (n & (n - 1)) == 0;
-  if (single_stage(n, true)) {
-    rs = reg_size(n) ? reg_radices(n) : choose_radices(n);
-  } else {
-    int64_t P;
-    if (!choose_split(n, P)) return -1;
-    std::vector<int> a = choose_radices(P), b = choose_radices(n / P);
-    rs = a;
-    rs.insert(rs.end(), b.begin(), b.end());
-    sp = (int)a.size();
-  }
+  int sp = 0, sp2 = 0;
+  if (!estimate_schedule(&shape, rs, sp, sp2)) return -1;
   for (int i = 0; i < (int)rs.size() && i < max; ++i) radices[i] = rs[i];
   if (split) *split = sp;
   return (int)rs.size();

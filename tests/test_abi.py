@@ -296,3 +296,26 @@ def test_schedule_cost_orders_stages_and_passes(lf):
     assert lf.schedule_cost(True, 4096, [4, 4, 4, 4, 4, 4], 0) > lf.schedule_cost(True, 4096, [16, 16, 16], 0)
     # HBM dominates: 32 N bytes at 6.5 TB/s per stage
     assert 32 * 4096 / 6.5e3 < one < 2 * 32 * 4096 / 6.5e3
+
+
+def test_planner_covers_the_reference_domain(lf):
+    """Every 13-smooth size of the reference's fast engine's domain (2 <= N <= 2^24,
+    bench-leanfftw.c:253-268) has a schedule: one, two or three kernel stages, each stage's
+    sub-problem within the 2048-point LDS tile."""
+    lim = 1 << 24
+    nums = {1}
+    for pr in (2, 3, 5, 7, 11, 13):
+        grown = set()
+        for n in nums:
+            m = n
+            while m * pr <= lim:
+                m *= pr
+                grown.add(m)
+        nums |= grown
+    nums = sorted(n for n in nums if n >= 2)
+    assert len(nums) == 9597
+    for n in nums:
+        rs, split = lf.schedule(n)
+        assert int(np.prod(rs)) == n
+    with pytest.raises(lf.MfftError):
+        lf.schedule(17)

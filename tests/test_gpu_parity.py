@@ -398,3 +398,53 @@ def test_fft_on_two_streams_same_shape(lf):
         torch.cuda.synchronize()
         for o, r in zip(outs, refs):
             assert torch.equal(o, r)
+
+
+# ------------------------------------------------------------------ three kernel stages
+
+
+@pytest.mark.parametrize("N,rs,passes", [(4096, [16, 16, 16], [1, 1, 1]), (360, [3, 4, 2, 3, 5], [2, 1, 2]),
+                                         (210, [5, 7, 2, 3], [1, 1, 2]), (65536, [8, 8, 4, 8, 8, 4], [2, 2, 2]),
+                                         (10368, [12, 8, 12, 9], [1, 2, 1])])
+@pytest.mark.parametrize("fwd", [True, False])
+def test_three_stage_forced_bit_exact(lf, oracle, N, rs, passes, fwd):
+    """Small sizes forced into three kernel stages (x -> T -> U -> y; the middle stage with
+    sub-problems along grid.y and transforms along grid.z): bit-exact against refTransform
+    with the same radices, out of place and in place, a batch of several transforms."""
+    hm = 5
+    x = oracle.lcg_input(N, seed=N ^ 77 ^ (fwd << 3), howmany=hm)
+    p = lf.Plan(N, lf.FORWARD if fwd else lf.BACKWARD, hm, radices=rs, passes=passes)
+    assert p.stages == passes
+    ref = oracle.ref_transform(fwd, rs, x)
+    assert bits_equal(run(lf, p, x), ref)
+    assert bits_equal(run(lf, p, x, inplace=True), ref)
+
+
+@pytest.mark.parametrize("N", [371293, 1 << 23])
+def test_three_stage_sizes_bit_exact(lf, oracle, N):
+    """Sizes no two-way split fits (13^5, 2^23) run as three stages by default, bit-exact
+    against the oracle (one forward transform each; its exact-integer tables take seconds)."""
+    p = lf.Plan(N, lf.FORWARD, 2)
+    assert len(p.stages) == 3, p.stages
+    x = oracle.lcg_input(N, seed=N % 1009, howmany=2)
+    assert bits_equal(run(lf, p, x)[1], oracle.ref_transform(True, p.radices, x[1]))
+
+
+@pytest.mark.parametrize("N", [1 << 24, 3 ** 14, 13 ** 6])
+def test_three_stage_largest_properties(lf, N):
+    """The largest sizes of the reference's fast domain (2^24 = 256 MiB per transform, 3^14,
+    13^6): round trip backward(forward(x)) = N x and Parseval."""
+    p = lf.Plan(N, lf.FORWARD, 1)
+    q = lf.Plan(N, lf.BACKWARD, 1)
+    assert len(p.stages) == 3
+    g = torch.Generator(device=DEV).manual_seed(N % 997)
+    x = (torch.rand(N, dtype=torch.float64, device=DEV, generator=g) - 0.5) + 1j * (
+        torch.rand(N, dtype=torch.float64, device=DEV, generator=g) - 0.5)
+    y = torch.empty_like(x)
+    z = torch.empty_like(x)
+    p.call(x, y)
+    q.call(y, z)
+    torch.cuda.synchronize()
+    assert ((z / N - x).abs().max() / x.abs().max()).item() < 1e-14 * math.log2(N)
+    ex, ey = (x.abs() ** 2).sum().item(), (y.abs() ** 2).sum().item()
+    assert abs(ey - N * ex) <= 1e-11 * N * ex
