@@ -1010,7 +1010,7 @@ double gpu_cost(bool fwd, int64_t N, const std::vector<int> &rs, int split) {
 // shape (tile G sub-problems on NT threads), or none (the register or stage kernel). The
 // planner's own schedule (ESTIMATE: the hand-measured radices and tile tables) is always a
 // candidate, so MEASURE keeps it unless something timed faster. Candidates run on scratch
-// buffers of up to 64 MB of the plan's layout; the winner is cached per (device, size,
+// buffers of up to 256 MB of the plan's layout (the real batch up to that size); the winner is cached per (device, size,
 // direction, batch class, layout) in a JSON-lines wisdom file.
 
 constexpr double kPruneCut = 1.5;
@@ -1289,11 +1289,43 @@ int runs_per_timing(double us) {
   return us >= 300.0 ? 1 : (int)std::min(32.0, std::ceil(300.0 / std::max(us, 1.0)));
 }
 
+// Fill `n` complex values at `d` with uniform[-0.5, 0.5) data from the reference's input LCG
+// (scripts/NativeBench.lean:31-37): a 1 MiB-element block made on the host, then doubled on the
+// device. The race times candidates on such data, not on zeros: on MI355X a plan's time on
+// zeros differs from its time on real data by 5-10%, and not by the same amount for every
+// plan (32768 points: 8 x 8 x 4 | 8 x 8 x 2 is the fastest on random data and the slowest of
+// four schedules on zeros).
+bool fill_uniform(double2 *d, int64_t n) {
+  const int64_t blk = std::min<int64_t>(n, int64_t(1) << 20);
+  std::vector<double2> h((size_t)blk);
+  uint64_t st = 0x9E3779B97F4A7C15ull;
+  auto next = [&st] {
+    st = st * 6364136223846793005ull + 1442695040888963407ull;
+    return (double)(st >> 11) / 9007199254740992.0 - 0.5;
+  };
+  for (auto &v : h) {
+    v.x = next();
+    v.y = next();
+  }
+  if (hipMemcpy(d, h.data(), (size_t)blk * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess)
+    return false;
+  for (int64_t done = blk; done < n;) {
+    const int64_t c = std::min(done, n - done);
+    if (hipMemcpy(d + done, d, (size_t)c * sizeof(double2), hipMemcpyDeviceToDevice) != hipSuccess)
+      return false;
+    done += c;
+  }
+  return true;
+}
+
 // Race the candidates on scratch buffers; returns false (and leaves `best` alone) when the
 // layout's span is too large to tune on scratch, or nothing could be timed.
 bool measure(const mfft_plan *shape, Cand &best) {
   const int64_t n = shape->n;
-  const int64_t hm = std::max<int64_t>(1, std::min<int64_t>(shape->howmany, (int64_t(1) << 22) / n));
+  // the plan's own batch up to 256 MB (2^24 elements): tile and XCD-remap choices depend on
+  // how many sub-problems a launch has, so a smaller sample can prefer a plan that is slower
+  // at the real batch (165375 points: 272 us ESTIMATE, 301 us for the choice timed at 64 MB)
+  const int64_t hm = std::max<int64_t>(1, std::min<int64_t>(shape->howmany, (int64_t(1) << 24) / n));
   auto span = [&](int64_t st, int64_t d) { return std::llabs(st) * (n - 1) + std::llabs(d) * (hm - 1) + 1; };
   const int64_t si = span(shape->istride, shape->idist), so = span(shape->ostride, shape->odist);
   if (si > (int64_t(1) << 25) || so > (int64_t(1) << 25)) return false;
@@ -1341,7 +1373,11 @@ bool measure(const mfft_plan *shape, Cand &best) {
     (void)hipFree(din);
     return false;
   }
-  (void)hipMemset(din, 0, si * sizeof(double2));
+  if (!fill_uniform(din, si)) {
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    return false;
+  }
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   (void)hipStreamCreate(&st);

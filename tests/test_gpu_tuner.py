@@ -159,3 +159,26 @@ def test_wisdom_tile_choice_is_obeyed(lf, oracle, tmp_path):
             assert bits_equal(run_plan(p, x), oracle.ref_transform(True, [8, 8, 8, 2], x))
     finally:
         lf.set_wisdom_path(None)
+
+
+@pytest.mark.parametrize("n", [10368, 75600, 165375, 32768, 262144])
+def test_measure_not_slower_at_full_batch(lf, tmp_path, n):
+    """At the benchmark's own batch (ceil(2^24 / n) transforms, 256 MB), where tile widths and
+    the XCD-aware order depend on the launch's sub-problem count, MEASURE (timed at that batch)
+    is no slower than ESTIMATE."""
+    lf.set_wisdom_path(str(tmp_path / "wisdom.jsonl"))
+    try:
+        hm = -(-(1 << 24) // n)
+        pe = lf.Plan(n, -1, hm)
+        pm = lf.Plan(n, -1, hm, flags=lf.MEASURE)
+        g = torch.Generator(device=DEV).manual_seed(n)
+        x = torch.complex(torch.rand(hm, n, dtype=torch.float64, device=DEV, generator=g),
+                          torch.rand(hm, n, dtype=torch.float64, device=DEV, generator=g))
+        y = torch.empty_like(x)
+        te = tm = float("inf")
+        for _ in range(4):
+            te = min(te, _time(pe, x, y, reps=2, k=4))
+            tm = min(tm, _time(pm, x, y, reps=2, k=4))
+        assert tm <= 1.03 * te + 1.0, (n, pe.radices, pe.stages, pm.radices, pm.stages, te, tm)
+    finally:
+        lf.set_wisdom_path(None)
