@@ -533,12 +533,30 @@ def stream_key():
     return (s.device.index, s.cuda_stream)
 
 
-def _cached_plan(n, sign, howmany):
-    key = (n, sign, howmany, stream_key())
-    p = _plan_cache.get(key)
-    if p is None:
-        p = Plan(n, sign, howmany)
-        _plan_cache[key] = p
+def _cached_plan(n, sign, howmany, cache=None):
+    """The plan of (n, sign, howmany) for the current stream, created on first use.
+
+    During a HIP graph capture no plan is created: making one uploads its tables with
+    synchronous copies, which a capture forbids. The capture takes the plan made for this
+    shape before it on another stream of the same device (torch captures on a stream of its
+    own), so the graph's replays share that plan's intermediate with its executions on that
+    stream and must not overlap them; with no such plan it raises MfftError before anything
+    is recorded."""
+    cache = _plan_cache if cache is None else cache
+    sk = stream_key()
+    key = (n, sign, howmany, sk)
+    p = cache.get(key)
+    if p is not None:
+        return p
+    if _capturing():
+        for (kn, ks, kh, (dev, _)), q in cache.items():
+            if (kn, ks, kh, dev) == (n, sign, howmany, sk[0]):
+                return q
+        raise MfftError(f"no plan of {n} points x {howmany} (sign {sign}) exists on device "
+                        f"{sk[0]}: plans cannot be created during a HIP graph capture, so run "
+                        "the step once before capturing it")
+    p = Plan(n, sign, howmany)
+    cache[key] = p
     return p
 
 

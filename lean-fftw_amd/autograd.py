@@ -29,14 +29,9 @@ _plans: dict = {}
 
 
 def _plan(n: int, sign: int, howmany: int):
-    # one plan per stream: a plan's intermediate must not be shared by executions that
-    # overlap on two streams
-    key = (n, sign, howmany, _lf().stream_key())
-    p = _plans.get(key)
-    if p is None:
-        p = _lf().Plan(n, sign, howmany)
-        _plans[key] = p
-    return p
+    # one plan per stream (a plan's intermediate must not be shared by executions that
+    # overlap on two streams); under a graph capture, the plan made before it
+    return _lf()._cached_plan(n, sign, howmany, _plans)
 
 
 class FFTFunction(torch.autograd.Function):

@@ -319,3 +319,29 @@ def test_planner_covers_the_reference_domain(lf):
         assert int(np.prod(rs)) == n
     with pytest.raises(lf.MfftError):
         lf.schedule(17)
+
+
+def test_plan_cache_under_capture(lf, monkeypatch):
+    """Outside a capture the torch-level cache makes one plan per (shape, stream); during a
+    capture it creates none (a plan's table uploads are synchronous copies, which a capture
+    forbids): it takes this shape's plan from another stream of the device, or raises."""
+    made = []
+    state = {"stream": (0, 111), "capturing": False}
+    monkeypatch.setattr(lf, "Plan", lambda n, sign, howmany: made.append((n, sign, howmany)) or object())
+    monkeypatch.setattr(lf, "stream_key", lambda: state["stream"])
+    monkeypatch.setattr(lf, "_capturing", lambda: state["capturing"])
+    cache = {}
+    a = lf._cached_plan(4096, -1, 8, cache)
+    assert lf._cached_plan(4096, -1, 8, cache) is a and len(made) == 1
+    state["stream"] = (0, 222)
+    b = lf._cached_plan(4096, -1, 8, cache)
+    assert b is not a and len(made) == 2
+    state.update(stream=(0, 333), capturing=True)
+    assert lf._cached_plan(4096, -1, 8, cache) in (a, b)
+    for shape in ((4096, 1, 8), (4096, -1, 16), (2048, -1, 8)):
+        with pytest.raises(lf.MfftError, match="capture"):
+            lf._cached_plan(*shape, cache)
+    state["stream"] = (1, 333)  # another device's plans are not taken
+    with pytest.raises(lf.MfftError):
+        lf._cached_plan(4096, -1, 8, cache)
+    assert len(made) == 2

@@ -400,6 +400,30 @@ def test_fft_on_two_streams_same_shape(lf):
             assert torch.equal(o, r)
 
 
+@pytest.mark.parametrize("N,hm", [(4096, 32), (65536, 16)])
+def test_fft_captured_on_torch_capture_stream(lf, N, hm):
+    """lf.fft inside torch.cuda.graph, which captures on a stream of its own: the capture takes
+    the plan made before it (creating one would need synchronous copies) and every replay, with
+    eager work in between, matches the eager result bit for bit."""
+    g = torch.Generator(device=DEV).manual_seed(N + 3)
+    x = torch.complex(torch.rand(hm, N, dtype=torch.float64, device=DEV, generator=g) - 0.5,
+                      torch.rand(hm, N, dtype=torch.float64, device=DEV, generator=g) - 0.5)
+    ref = lf.fft(x)
+    torch.cuda.synchronize()
+    before = len(lf._plan_cache)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        out = lf.fft(x)
+    assert len(lf._plan_cache) == before
+    for i in range(3):
+        out.zero_()
+        junk = torch.randn(1 << 20, device=DEV) * 2.0  # eager work between replays
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), i
+        del junk
+
+
 # ------------------------------------------------------------------ three kernel stages
 
 
