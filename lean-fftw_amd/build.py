@@ -30,6 +30,7 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]
 SOURCES = [
     ("mfft_kernels.hip", True),
     ("mfft_reg.hip", True),
+    ("mfft_split.hip", True),
     ("mfft_tile.hip", True),
     ("mfft_tile2d.hip", True),
     *((f"mfft_tile_gen{i}.hip", True) for i in range(8)),

@@ -135,6 +135,13 @@ int reg_stage_g(int L, const int *radices, int nradix, int gmax);
 hipError_t launch_reg_stage(bool fwd, const StageArgs &a, int64_t ntrans, const double2 *in,
                             double2 *out, const double2 *tw, hipStream_t stream);
 int reg_elems(int64_t n);  // elements per thread of the register-resident kernel at size n
+// Split register kernels (mfft_split.hip): one transform per workgroup as column and row
+// transforms inside the workgroup, for the radix sequences in its table (non-null if there is
+// one); launch_split returns false when the stage is not one it runs (element maps, strides,
+// several transforms per workgroup), else launches it and sets *err.
+const void *split_schedule(int64_t n, const int *radices, int nradix);
+bool launch_split(bool fwd, const StageArgs &a, const double2 *in, double2 *out,
+                  const double2 *tw, hipStream_t stream, hipError_t *err);
 hipError_t launch_reg(bool fwd, const StageArgs &a, const double2 *in, double2 *out,
                       const double2 *tw, hipStream_t stream);
 

@@ -191,17 +191,20 @@ std::vector<int> choose_radices(int64_t L, bool mid_only = false) {
 // elements per thread: 125 us per 256 MB batch against 180 for radix 4 only). Plain complex
 // plans (`fuse` false) take radix 16 where it measured faster on MI355X (256 MB batches):
 // 1024 = 16 x 8 x 8 in a tile kernel (78 us against 85 for 8 x 8 x 8 x 2), 2048 = 16 x 8 x 16
-// in a tile kernel (80.5 against 82.9 for 8 x 8 x 8 x 4), 8192 =
-// 16 x 16 x 8 x 4 at 16 elements per thread (98-101 against 106-110). (4096 = 16^3 at 256
+// in a tile kernel (80.5 against 82.9 for 8 x 8 x 8 x 4), and the split kernel's schedules
+// (mfft_split.hip: wave-local column and row passes, one workgroup-wide hand-over) at 8192 =
+// 16 x 4 | 16 x 8 (101-102 us against 109-112 for 16 x 16 x 8 x 4 in the fixed kernel) and
+// 16384 = 16 x 8 | 16 x 8 (111-113 against 129-135 for 16 x 16 x 16 x 4). (4096 = 16^3 at 256
 // threads per transform measured 85-92 against 92, but it halves the threads of a batch and
-// its weighted kernel took 34 us against 19 on a 512-transform batch.) The parts of
-// real-data plans (`fuse`) keep the 8-per-thread schedules, into whose loads and stores the
-// real-data maps fuse.
+// its weighted kernel took 34 us against 19 on a 512-transform batch; 4096 = 8 x 8 | 8 x 8 in
+// the split kernel measured 92-94 against 91-93, four workgroups per CU already overlapping
+// each other's barriers.) The parts of real-data plans (`fuse`) keep the fixed schedules,
+// into whose loads and stores the real-data maps fuse.
 std::vector<int> reg_radices(int64_t n, bool fuse = false) {
-  if (n == 16384) return {16, 16, 16, 4};
+  if (n == 16384) return fuse ? std::vector<int>{16, 16, 16, 4} : std::vector<int>{16, 8, 16, 8};
   if (!fuse && n == 1024) return {16, 8, 8};
   if (!fuse && n == 2048) return {16, 8, 16};
-  if (!fuse && n == 8192) return {16, 16, 8, 4};
+  if (!fuse && n == 8192) return {16, 4, 16, 8};
   std::vector<int> rs;
   double c;
   best_radices(n, any_radix, rs, c, 8);

@@ -598,7 +598,7 @@ int reg_elems(int64_t n) { return n > 8192 ? 16 : 8; }
 
 bool reg_supported(int64_t n, const int *radices, int nradix) {
   if (n < 32 || n > 16384 || (n & (n - 1)) || nradix < 1 || nradix > MAX_PASSES) return false;
-  if (fixed_schedule(n, radices, nradix)) return true;
+  if (fixed_schedule(n, radices, nradix) || split_schedule(n, radices, nradix)) return true;
   for (int i = 0; i < nradix; ++i) {
     const int r = radices[i];
     if (reg_elems(n) == 16 ? r != 4 : (r != 2 && r != 4 && r != 8)) return false;
@@ -609,6 +609,10 @@ bool reg_supported(int64_t n, const int *radices, int nradix) {
 hipError_t launch_reg(bool fwd, const StageArgs &a, const double2 *in, double2 *out,
                       const double2 *tw, hipStream_t stream) {
   if (a.nsub <= 0) return hipSuccess;
+  {
+    hipError_t e;
+    if (launch_split(fwd, a, in, out, tw, stream, &e)) return e;
+  }
   int rs[MAX_PASSES];
   for (int p = 0; p < a.npass; ++p) rs[p] = a.pass[p].R;
   if (const FixedSchedule *f = fixed_schedule(a.L, rs, a.npass)) {
@@ -629,6 +633,12 @@ hipError_t launch_reg(bool fwd, const StageArgs &a, const double2 *in, double2 *
     return hipLaunchKernel(k, dim3((unsigned)nblk), dim3(BT), args, lds, stream);
   }
   const int E = reg_elems(a.L), T = a.L / E;
+  // the run-time dispatched kernel holds radices 2, 4, 8 at 8 elements per thread and radix 4
+  // at 16 (a schedule only the split kernel runs, asked for with element maps, is refused)
+  for (int p = 0; p < a.npass; ++p) {
+    const int r = a.pass[p].R;
+    if (E == 16 ? r != 4 : (r != 2 && r != 4 && r != 8)) return hipErrorInvalidValue;
+  }
   const int BT = T < 256 ? 256 : T, G = BT / T;  // as reg_block<T>()
   const size_t lds = E == 8 ? (size_t)G * lds_len(a.L) * sizeof(double2)
                             : (size_t)reg_lds_doubles(a.L) * sizeof(double);

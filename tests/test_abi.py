@@ -144,8 +144,8 @@ def test_missing_library_fails_loudly(lf, tmp_path, monkeypatch):
 
 def test_planner_schedules_are_valid(lf):
     """Every schedule multiplies to N, uses radices 2..16 with prime factors <= 13, and its
-    kernel stages fit the LDS tiles or, for N = 16384, the register-resident kernel's fixed
-    schedule 16 x 16 x 16 x 4. No GPU needed: the planner query allocates nothing."""
+    kernel stages fit the LDS tiles or, for N = 16384, the split register kernel's schedule
+    16 x 8 | 16 x 8. No GPU needed: the planner query allocates nothing."""
     import math
 
     w = lf.workloads
@@ -159,7 +159,7 @@ def test_planner_schedules_are_valid(lf):
             P = math.prod(rs[:split])
             assert P <= 2048 and n // P <= 2048, (n, rs, split)
         elif n > 8192:
-            assert n == 16384 and rs == [16, 16, 16, 4], (n, rs)
+            assert n == 16384 and rs == [16, 8, 16, 8], (n, rs)
     with pytest.raises(lf.MfftError):
         lf.schedule(17)
 

@@ -110,7 +110,8 @@ def test_stage_splits_bit_exact(lf, oracle, N, rs, split):
                                   (8192, [8, 4, 4, 4, 4, 4]), (16384, [16, 16, 16, 4]),
                                   (16384, [8, 8, 8, 8, 4]), (16384, [4, 4, 4, 4, 4, 4, 4]),
                                   (4096, [4, 8, 2, 8, 8]), (1024, [2, 4, 8, 4, 4]),
-                                  (8192, [16, 16, 8, 4])])
+                                  (8192, [16, 16, 8, 4]), (8192, [16, 4, 16, 8]),
+                                  (16384, [16, 8, 16, 8])])
 def test_register_schedules_bit_exact(lf, oracle, N, rs):
     """The register-resident kernel's fixed (compile-time) schedules and its run-time
     dispatched ones, both directions, in and out of place."""
@@ -190,6 +191,36 @@ def test_weighted_transform_bit_exact(lf, oracle):
     xw = x * w  # the same binary64 product the kernel forms on load
     ref = oracle.ref_transform(True, plan.radices, xw) * w
     assert bits_equal(out.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("N", [8192, 16384])
+def test_split_kernel_weights_and_partials(lf, oracle, N):
+    """The split register kernel (the default schedules of 8192 and 16384 points) with input
+    and output weights, each alone and together, and with per-workgroup sums of |out|^2."""
+    hm = 3
+    x = oracle.lcg_input(N, seed=19 + N, howmany=hm)
+    w = np.random.default_rng(N).uniform(0, 1, N)
+    plan = lf.Plan(N, lf.FORWARD, hm)
+    assert plan.radices == ([16, 4, 16, 8] if N == 8192 else [16, 8, 16, 8])
+    xd = to_dev(x)
+    wd = torch.from_numpy(w).to(DEV)
+    for pre, post in ((wd, None), (None, wd), (wd, wd)):
+        out = torch.empty_like(xd)
+        plan.call_weighted(xd, out, pre=pre, post=post)
+        torch.cuda.synchronize()
+        ref = oracle.ref_transform(True, plan.radices, x * w if pre is not None else x)
+        if post is not None:
+            ref = ref * w
+        assert bits_equal(out.cpu().numpy(), ref), (pre is not None, post is not None)
+    parts = torch.full((plan.partials,), float("nan"), dtype=torch.float64, device=DEV)
+    out = torch.empty_like(xd)
+    plan.call_weighted_sumsq(xd, out, parts, post=wd)
+    torch.cuda.synchronize()
+    ref = oracle.ref_transform(True, plan.radices, x) * w
+    assert bits_equal(out.cpu().numpy(), ref)
+    assert plan.partials == hm and not torch.isnan(parts).any()
+    rows = (ref.real ** 2 + ref.imag ** 2).sum(axis=1)
+    assert np.allclose(parts.cpu().numpy(), rows, rtol=1e-13, atol=0)
 
 
 def test_spec_plan(lf, oracle):
