@@ -149,14 +149,15 @@ def test_partials_every_kernel_shape(lf, oracle, N, B):
     assert abs(parts.sum().item() - tot) <= 1e-12 * tot
 
 
-@pytest.mark.skip(reason="host segfault inside torch's CUDAGraph.capture_end when the autograd "
-                         "engine's backward ran under stream capture; two hazards on that path "
-                         "were fixed (launch status from the thread's last error, plan frees "
-                         "during a capture) but neither is shown to be the cause, so the case "
-                         "stays off the GPU")
 def test_graph_captured_fwd_bwd_replays(lf):
     """The fused loss and its backward captured in a HIP graph: every replay, with eager work
-    in between, matches the eager result bit for bit."""
+    in between, matches the eager result bit for bit.
+
+    Each step keeps only the detached loss. Keeping the loss with its graph would keep x's
+    gradient accumulator alive from the first eager step, and with it that step's stream (the
+    legacy default stream): the captured backward would then synchronise the capture stream
+    with the legacy stream (the engine's producer/consumer and leaf-stream events), which a
+    global-mode capture forbids."""
     B, N = 64, 4096
     g = torch.Generator(device=DEV).manual_seed(77)
     x = torch.complex(torch.rand(B, N, dtype=torch.float64, device=DEV, generator=g) - 0.5,
@@ -169,7 +170,7 @@ def test_graph_captured_fwd_bwd_replays(lf):
         x.grad = None
         loss = lf.weighted_power_loss(x, w)
         loss.backward()
-        out["loss"] = loss
+        out["loss"] = loss.detach()
 
     step()
     ref_loss = out["loss"].detach().clone()
