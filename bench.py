@@ -438,8 +438,9 @@ def capture_graph(torch, fn, warm=3):
 
 def bench_autograd(args, torch, lf):
     """Config 5: loss = sum(|w * FFT(x)|^2), x (512, 4096) complex128; forward alone vs
-    forward + backward (the hand-written adjoint). Both steps are timed as captured HIP
-    graphs (GPU cost) and eagerly (including Python and autograd-engine overhead)."""
+    forward + backward (the hand-written adjoint) through the autograd Function. Both steps
+    are timed as captured HIP graphs (GPU cost) and eagerly (including Python and
+    autograd-engine launch overhead)."""
     B, N = 512, 4096
     x = uniform_complex(torch, (B, N), 55).requires_grad_(True)
     g = torch.Generator(device="cuda:0").manual_seed(56)
@@ -457,38 +458,19 @@ def bench_autograd(args, torch, lf):
     steps = args.steps * 4
     t_f = time_steps(torch, fwd, steps, args.warmup)
     t_fb = time_steps(torch, fwd_bwd, steps, args.warmup)
-    # GPU cost: the forward loss and the backward the Function runs (the real weight 2 g w,
-    # then the adjoint kernel on the saved spectrum), each captured in a HIP graph
-    fplan = lf.Plan(N, lf.FORWARD, B)
-    bplan = lf.Plan(N, lf.BACKWARD, B)
-    xd = x.detach()
-    y = torch.empty_like(xd)
-    gx = torch.empty_like(xd)
-    parts = torch.empty(fplan.partials, dtype=torch.float64, device="cuda:0")
-    scale = torch.empty_like(w)
-    gout = torch.ones((), dtype=torch.float64, device="cuda:0")
-    two = torch.full((), 2.0, dtype=torch.float64, device="cuda:0")
-
-    def g_fwd():
-        fplan.call_weighted_sumsq(xd, y, parts, post=w)
-        parts.sum()
-
-    def g_bwd():
-        torch.mul(w, two * gout, out=scale)
-        bplan.call_weighted(y, gx, pre=scale)
-
-    # K steps per graph, so that a replay (about 30 us of GPU work per step) is not bound by
-    # the host's graph launch
+    # GPU cost: the same two steps, autograd engine included, captured in HIP graphs; K steps
+    # per graph, so that a replay (about 30 us of GPU work per step) is not bound by the
+    # host's graph launch. No step keeps its graph alive into the next one (the capture must
+    # not inherit an eager step's gradient accumulator, which belongs to the default stream).
     K = 16
 
     def k_fwd():
         for _ in range(K):
-            g_fwd()
+            fwd()
 
     def k_fwd_bwd():
         for _ in range(K):
-            g_fwd()
-            g_bwd()
+            fwd_bwd()
 
     gf = capture_graph(torch, k_fwd)
     gfb = capture_graph(torch, k_fwd_bwd)
