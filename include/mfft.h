@@ -197,6 +197,17 @@ void mfft_destroy_ragged(mfft_ragged *p);
 int mfft_execute_weighted(const mfft_plan *p, const void *in, void *out, const double *pre,
                           const double *post, void *stream);
 
+/* As mfft_execute_weighted with input weights pre[n] (required) scaled by a device scalar:
+ * element e of every transform is multiplied by (factor * scale[0]) * pre[e], the weight
+ * formed in the kernel in that order, so the result is bit-identical to passing the vector
+ * (factor * scale[0]) * pre computed elementwise on the device. For the adjoint of the
+ * differentiable loss, whose weight 2 g w holds the upstream gradient g: a value on the
+ * device that the host never reads, so no elementwise launch or host sync is needed to form
+ * it. No counterpart in the reference. */
+int mfft_execute_weighted_scaled(const mfft_plan *p, const void *in, void *out,
+                                 const double *pre, const double *scale, double factor,
+                                 const double *post, void *stream);
+
 /* As mfft_execute_weighted, and every workgroup of the last kernel stage also writes the sum
  * of |out|^2 over the elements it stored to partials[i], 0 <= i < mfft_plan_partials(p):
  * the sum of the partials is sum |out|^2 (a deterministic fixed-order reduction). For the

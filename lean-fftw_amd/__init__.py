@@ -69,6 +69,8 @@ def lib():
                 "mfft_execute_spec": (ctypes.c_int, [vp, vp, vp, vp]),
                 "mfft_execute_weighted": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
                 "mfft_execute_weighted_sumsq": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+                "mfft_execute_weighted_scaled": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_double,
+                                                                vp, vp]),
                 "mfft_plan_partials": (i64, [vp]),
                 "mfft_destroy_plan": (None, [vp]),
                 "mfft_plan_radices": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
@@ -304,6 +306,29 @@ class Plan:
         _check(lib().mfft_execute_weighted(self._h, ctypes.c_void_p(inp.data_ptr()),
                                            ctypes.c_void_p(out.data_ptr()), ptr(pre), ptr(post),
                                            _stream_ptr(stream)), "mfft_execute_weighted")
+        return out
+
+    def call_weighted_scaled(self, inp, out, pre, scale, factor=1.0, post=None, stream=None):
+        """out <- diag(post) F diag((factor * scale[0]) * pre) inp: the input weights times a
+        scalar that stays on the device (a float64 CUDA tensor of one element), the product
+        formed in the kernel."""
+        self._check_tensor(inp, "input")
+        self._check_tensor(out, "output")
+        import torch
+
+        for w in (pre, post):
+            if w is not None and (w.dtype != torch.float64 or not w.is_cuda or w.numel() < self.n):
+                raise TypeError("weights must be float64 CUDA tensors of length n")
+        if pre is None:
+            raise TypeError("call_weighted_scaled needs input weights")
+        if scale.dtype != torch.float64 or not scale.is_cuda or scale.numel() != 1:
+            raise TypeError("scale must be a float64 CUDA tensor of one element")
+        ptr = lambda w: ctypes.c_void_p(w.data_ptr()) if w is not None else None  # noqa: E731
+        _check(lib().mfft_execute_weighted_scaled(self._h, ctypes.c_void_p(inp.data_ptr()),
+                                                  ctypes.c_void_p(out.data_ptr()), ptr(pre),
+                                                  ptr(scale), float(factor), ptr(post),
+                                                  _stream_ptr(stream)),
+               "mfft_execute_weighted_scaled")
         return out
 
     @property

@@ -77,7 +77,8 @@ class WeightedPowerLoss(torch.autograd.Function):
     Forward: one kernel writes y = w * F x and, in the same stores, per-workgroup partial
     sums of |y|^2 (a fixed-order reduction, so the loss is deterministic); the loss is the
     sum of those partials. Backward: grad_x = F^H (2 g w * y), g = d(out)/d(loss): one
-    adjoint kernel reading the saved y with the real weight 2 g w fused into its load.
+    adjoint kernel reading the saved y with the real weight (2 g) w formed in its load from
+    w and the device scalar g (no elementwise launch for the weight).
     """
 
     @staticmethod
@@ -100,9 +101,9 @@ class WeightedPowerLoss(torch.autograd.Function):
     def backward(ctx, g):
         y, w = ctx.saved_tensors
         n, howmany = ctx.meta
-        scale = (2.0 * g).to(torch.float64) * w
+        g = g.to(device=y.device, dtype=torch.float64).reshape(1).contiguous()
         out = torch.empty_like(y)
-        _plan(n, 1, howmany).call_weighted(y, out, pre=scale)
+        _plan(n, 1, howmany).call_weighted_scaled(y, out, pre=w, scale=g, factor=2.0)
         return out, None
 
 

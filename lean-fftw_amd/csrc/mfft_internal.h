@@ -99,6 +99,10 @@ struct StageArgs {
   const double *pre;
   const double *post;
   int64_t pre_g, pre_e, post_g, post_e;
+  // optional scalar factor of the input weights: with pre_s set, the input weight is
+  // (pre_k * pre_s[0]) * pre[...] (pre_s a device value, pre_k a host constant)
+  const double *pre_s;
+  double pre_k;
   // optional: each workgroup writes the sum of |out|^2 of its stores to
   // sumsq[blockIdx.y * gridDim.x + blockIdx.x] (last stage of a plan only)
   double *sumsq;

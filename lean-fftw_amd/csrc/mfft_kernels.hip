@@ -23,6 +23,7 @@ __global__ __launch_bounds__(MAXT) void stage_kernel(const double2 *in, double2 
               a.in_e, a.pre ? a.pre + g0 * a.pre_g : nullptr, a.pre_g, a.pre_e, a.cached_in};
   HbmSide hout{out + z * a.out_z + h * a.out_h + g0 * a.out_g, nullptr, a.out_g, a.out_e,
                a.post ? a.post + g0 * a.post_g : nullptr, a.post_g, a.post_e, a.cached_out};
+  if (a.pre_s) hin.wk = a.pre_k * a.pre_s[0];
   // (the run-time-sided pass body of run_stage_compact measured the same here: MI355X, mixed
   // sizes on this kernel 2.91 / 2.92 TB/s)
   const double part = run_stage<FWD, RSET, EPT, SQ>(lds, hin, hout, threadIdx.x, blockDim.x, a.G, gcount, a.L,

@@ -97,8 +97,9 @@ __device__ __forceinline__ bool stage_block(int xcd, int64_t nsub, int G, int &b
 
 // Where a pass reads or writes in HBM: element e of sub-problem g of the tile is
 // base[off(g) + e * stride], off(g) = list ? list[g] : g * gstride, optionally scaled by a
-// real weight wt[g * wg + e * wstride] (weights are shared across the batch). With SQ the
-// last pass also returns the sum of |x|^2 of the elements the thread stored.
+// real weight wt[g * wg + e * wstride] (weights are shared across the batch); on the input
+// side the weight is wk * wt[...] (wk = 1 unless the input weights carry a scalar factor).
+// With SQ the last pass also returns the sum of |x|^2 of the elements the thread stored.
 struct HbmSide {
   double2 *base;
   const int64_t *list;
@@ -106,6 +107,7 @@ struct HbmSide {
   const double *wt;
   int64_t wg, wstride;
   int cached = 0;  // ordinary loads / stores on this side instead of non-temporal ones
+  double wk = 1.0;
 };
 
 __device__ __forceinline__ int64_t hbm_off(const HbmSide &s, int g) {
@@ -163,7 +165,7 @@ __device__ __forceinline__ double pass_body(double2 *buf, const HbmSide &hin, co
           const int64_t ws = (int64_t)LR * hin.wstride;
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            const double v = w[r * ws];
+            const double v = hin.wk * w[r * ws];
             x[u][r].x = x[u][r].x * v;
             x[u][r].y = x[u][r].y * v;
           }
@@ -248,7 +250,7 @@ __device__ __forceinline__ void tile_to_lds(double2 *buf, const HbmSide &h, int 
       v[u] = ld_stream(h.base + f);
       if (h.wt) {
         const int g = (int)fdiv((uint32_t)f, m_L);
-        const double w = h.wt[(int64_t)g * h.wg + (int64_t)(f - g * L) * h.wstride];
+        const double w = h.wk * h.wt[(int64_t)g * h.wg + (int64_t)(f - g * L) * h.wstride];
         v[u].x = v[u].x * w;
         v[u].y = v[u].y * w;
       }

@@ -738,14 +738,18 @@ int work_for(const mfft_plan *p, const mfft::Workspace *ws, Work &w) {
   return MFFT_SUCCESS;
 }
 
+// pre_s / pre_k: optional scalar factor of the input weights (StageArgs::pre_s, pre_k).
 int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
-        const double *post, double *sumsq, hipStream_t st, const mfft::Workspace *ws = nullptr) {
+        const double *post, double *sumsq, hipStream_t st, const mfft::Workspace *ws = nullptr,
+        const double *pre_s = nullptr, double pre_k = 1.0) {
   if (p->howmany == 0) return MFFT_SUCCESS;
   if (!in || !out) return MFFT_ERR_INVALID;
   const bool fwd = p->sign < 0;
   if (p->stages.size() == 1) {
     StageArgs a = p->stages[0];
     a.pre = pre;
+    a.pre_s = pre_s;
+    a.pre_k = pre_k;
     a.post = post;
     a.sumsq = sumsq;
     if (p->outer == 1)
@@ -771,6 +775,8 @@ int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
       const int64_t hc = std::min(w.chunk, p->howmany - h0);
       StageArgs A = p->stages[0], B = p->stages[1], C = p->stages[2];
       A.pre = pre;
+      A.pre_s = pre_s;
+      A.pre_k = pre_k;
       C.post = post;
       C.sumsq = sumsq ? sumsq + h0 * nxc : nullptr;
       if (mfft::launch_stage(fwd, A, hc, in + h0 * p->idist, T, p->d_tw, st) != hipSuccess ||
@@ -805,6 +811,8 @@ int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
     const int64_t hc = std::min(w.chunk, p->howmany - h0);
     StageArgs A = p->stages[0], B = p->stages[1];
     A.pre = pre;
+    A.pre_s = pre_s;
+    A.pre_k = pre_k;
     B.post = post;
     B.sumsq = sumsq ? sumsq + h0 * nxb : nullptr;
     if (mfft::launch_stage(fwd, A, hc, in + h0 * p->idist, w.ptr, p->d_tw, st) != hipSuccess)
@@ -1746,6 +1754,16 @@ int mfft_execute_weighted(const mfft_plan *p, const void *in, void *out, const d
   if (p->kind != 0) return MFFT_ERR_UNSUPPORTED;
   if ((pre || post) && (p->istride != 1 || p->ostride != 1)) return MFFT_ERR_UNSUPPORTED;
   return run(p, (const double2 *)in, (double2 *)out, pre, post, nullptr, (hipStream_t)stream);
+}
+
+int mfft_execute_weighted_scaled(const mfft_plan *p, const void *in, void *out,
+                                 const double *pre, const double *scale, double factor,
+                                 const double *post, void *stream) {
+  if (!p || !pre || !scale) return MFFT_ERR_INVALID;
+  if (p->kind != 0) return MFFT_ERR_UNSUPPORTED;
+  if (p->istride != 1 || p->ostride != 1) return MFFT_ERR_UNSUPPORTED;
+  return run(p, (const double2 *)in, (double2 *)out, pre, post, nullptr, (hipStream_t)stream,
+             nullptr, scale, factor);
 }
 
 int64_t mfft_plan_partials(const mfft_plan *p) {

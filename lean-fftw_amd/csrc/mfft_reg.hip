@@ -155,9 +155,10 @@ __device__ __forceinline__ void reg_load(double2 (&v)[E], const StageArgs &a, co
       v[m] = valid ? load_io(a, in, base, (int64_t)(t + T * m) * a.in_e) : make_double2(0.0, 0.0);
   }
   if (a.pre) {
+    const double wk = a.pre_s ? a.pre_k * a.pre_s[0] : 1.0;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      const double w = a.pre[(int64_t)(t + T * m) * a.pre_e];
+      const double w = wk * a.pre[(int64_t)(t + T * m) * a.pre_e];
       v[m].x = v[m].x * w;
       v[m].y = v[m].y * w;
     }
@@ -487,9 +488,10 @@ __global__ __launch_bounds__(L / E * G, 4) void stage_reg_kernel(const double2 *
     }
     if (a.pre) {
       const double *w = a.pre + (g0 + (ok ? gi : 0)) * a.pre_g;
+      const double wk = a.pre_s ? a.pre_k * a.pre_s[0] : 1.0;
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        const double c = w[(int64_t)(ti + T * m) * a.pre_e];
+        const double c = wk * w[(int64_t)(ti + T * m) * a.pre_e];
         v[m].x = v[m].x * c;
         v[m].y = v[m].y * c;
       }

@@ -164,9 +164,10 @@ __device__ __forceinline__ void split_load(double2 (&v)[E], const double2 *src, 
   for (int m = 0; m < E; ++m) v[m] = ld_stream(s + Q * TC * m);
   if constexpr (PRE) {
     const double *w = a.pre + (int64_t)(q + Q * tc) * a.pre_e;
+    const double wk = a.pre_s ? a.pre_k * a.pre_s[0] : 1.0;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      const double c = w[(int64_t)(Q * TC * m) * a.pre_e];
+      const double c = wk * w[(int64_t)(Q * TC * m) * a.pre_e];
       v[m].x = v[m].x * c;
       v[m].y = v[m].y * c;
     }

@@ -78,7 +78,8 @@ def test_gradient_central_differences():
 
 
 def test_gradient_matches_closed_form(lf, oracle):
-    """grad_x = F^H (2 w^2 F x): the backward transform of the weighted forward output."""
+    """grad_x = F^H (2 w^2 F x): the backward transform of the weighted forward output, bit
+    for bit (the adjoint's weight (2 g) w with g = 1 is 2 w, and y (2 w) = (2 y) w exactly)."""
     B, N = 4, 4096
     x = oracle.lcg_input(N, seed=51, howmany=B)
     w = np.random.default_rng(9).uniform(0, 1, N)
@@ -91,6 +92,45 @@ def test_gradient_matches_closed_form(lf, oracle):
     ref = oracle.ref_transform(False, rsb, (2.0 * y) * w)
     got = xd.grad.cpu().numpy()
     assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+    assert bits_equal(got, ref)
+
+
+@pytest.mark.parametrize("N,B", [(16, 300), (1000, 3), (4096, 8), (16384, 2), (65536, 2)])
+def test_scaled_input_weights_bit_exact(lf, oracle, N, B):
+    """Input weights times a device scalar (mfft_execute_weighted_scaled): equal, bit for bit,
+    to the unscaled entry point with the weight vector (2 s) w formed on the device, and to
+    the oracle (the generic stage kernel with staged copies and without, the fixed register
+    kernel, the split kernel, a two-stage plan)."""
+    x = oracle.lcg_input(N, seed=70 + N, howmany=B)
+    w = np.random.default_rng(N + 1).uniform(0, 1, N)
+    s = 0.37
+    p = lf.Plan(N, 1, B)
+    xd = torch.from_numpy(x).to(DEV)
+    wd = torch.from_numpy(w).to(DEV)
+    sd = torch.tensor([s], dtype=torch.float64, device=DEV)
+    y1 = torch.empty_like(xd)
+    p.call_weighted_scaled(xd, y1, pre=wd, scale=sd, factor=2.0)
+    y2 = torch.empty_like(xd)
+    p.call_weighted(xd, y2, pre=(2.0 * sd) * wd)
+    torch.cuda.synchronize()
+    assert bits_equal(y1.cpu().numpy(), y2.cpu().numpy())
+    ref = oracle.ref_transform(False, p.radices, x * ((2.0 * s) * w))
+    assert bits_equal(y1.cpu().numpy(), ref)
+
+
+def test_loss_gradient_with_upstream_scale(lf, oracle):
+    """d(c loss)/dx for an upstream gradient c: the adjoint's weight (2 c) w is formed in the
+    kernel from the device scalar c, bit-exact against the oracle."""
+    B, N = 4, 4096
+    x = oracle.lcg_input(N, seed=53, howmany=B)
+    w = np.random.default_rng(10).uniform(0, 1, N)
+    xd = torch.from_numpy(x).to(DEV).requires_grad_(True)
+    loss = lf.weighted_power_loss(xd, torch.from_numpy(w).to(DEV))
+    c = 0.731
+    (gx,) = torch.autograd.grad(loss, xd, torch.tensor(c, dtype=torch.float64, device=DEV))
+    y = oracle.ref_transform(True, lf.Plan(N, -1, B).radices, x) * w
+    ref = oracle.ref_transform(False, lf.Plan(N, 1, B).radices, y * ((2.0 * c) * w))
+    assert bits_equal(gx.cpu().numpy(), ref)
 
 
 def test_fused_loss_matches_unfused(lf, oracle):
