@@ -168,7 +168,9 @@ mfft_plan *mfft_plan_with_stages(int64_t n, int sign, int64_t howmany, const int
 /* Ragged batch: `count` transforms of sizes sizes[i], packed back to back (transform i
  * starts at element sum_{j<i} sizes[j]), one direction, executed as two persistent launches
  * from size-sorted work queues: every stage-A and single-stage tile, then every stage-B tile
- * (the launch boundary orders the stages; a HIP graph replays both as one unit). in != out.
+ * (the launch boundary orders the stages; a HIP graph replays both as one unit), every
+ * workgroup taking tiles from one shared queue per launch. in != out. The plan's queue heads
+ * are device state: executes of one ragged plan must not overlap (one stream, or ordered).
  * No counterpart in the reference (it has no batched fast path, bench-leanfftw.c:257). */
 mfft_ragged *mfft_plan_ragged(int64_t count, const int64_t *sizes, int sign, unsigned flags);
 int mfft_execute_ragged(const mfft_ragged *p, const void *in, void *out, void *stream);
@@ -178,16 +180,20 @@ int mfft_execute_ragged(const mfft_ragged *p, const void *in, void *out, void *s
 int mfft_ragged_class_plan(const mfft_ragged *p, int64_t n, int *radices, int max, int *split);
 /* Synchronise `stream`: MFFT_SUCCESS when the enqueued launches completed, else MFFT_ERR_HIP. */
 int mfft_ragged_status(const mfft_ragged *p, void *stream);
-/* Host only (tests): the two work queues of a ragged batch of `count` transforms, built from
+/* Host only (tests): the work queues of a ragged batch of `count` transforms, built from
  * explicit class shapes -- per class, ascending distinct sizes, six values: n, stages (1 or
- * 2), P, Q (n = P Q), tile width of stage A (or of the single stage), of stage B. Writes
- * five values per tile (class, stage, count, base, sub0) for at most max_tiles tiles, the
- * first *n_first of them the first launch's, and the transform offsets single-stage tiles
- * list (base indexes them). Returns the number of tiles, -1 for inconsistent shapes or a
- * size without a class. No counterpart in the reference. */
+ * 2), P, Q (n = P Q), tile width of stage A (or of the single stage), of stage B -- with
+ * `nqueue` queues per launch (1..64; the batch plans use one). Writes five values
+ * per tile (class, stage, count, base, sub0) for at most max_tiles tiles, the first *n_first
+ * of them the first launch's; qoff (2 (nqueue + 1) values, may be NULL) receives the queue
+ * boundaries, queue q of the first launch at tiles [qoff[q], qoff[q + 1]) and of the second
+ * at [qoff[nqueue + 1 + q], qoff[nqueue + 2 + q]); offs receives the transform offsets
+ * single-stage tiles list (base indexes them). Returns the number of tiles, -1 for
+ * inconsistent shapes or a size without a class. No counterpart in the reference. */
 int64_t mfft_ragged_queue(int64_t count, const int64_t *sizes, int64_t nclass,
-                          const int64_t *shapes, int64_t *tiles, int64_t max_tiles,
-                          int64_t *n_first, int64_t *offs, int64_t max_offs);
+                          const int64_t *shapes, int64_t nqueue, int64_t *tiles,
+                          int64_t max_tiles, int64_t *n_first, int64_t *qoff, int64_t *offs,
+                          int64_t max_offs);
 void mfft_destroy_ragged(mfft_ragged *p);
 
 /* Fused weighted transform for the differentiable path: out = diag(post) F (diag(pre) in),

@@ -83,8 +83,10 @@ def lib():
                 "mfft_ragged_class_plan": (ctypes.c_int, [vp, i64, ctypes.POINTER(ctypes.c_int),
                                                           ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
                 "mfft_ragged_queue": (i64, [i64, ctypes.POINTER(ctypes.c_int64), i64,
-                                            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
-                                            i64, ctypes.POINTER(ctypes.c_int64),
+                                            ctypes.POINTER(ctypes.c_int64), i64,
+                                            ctypes.POINTER(ctypes.c_int64), i64,
+                                            ctypes.POINTER(ctypes.c_int64),
+                                            ctypes.POINTER(ctypes.c_int64),
                                             ctypes.POINTER(ctypes.c_int64), i64]),
                 "mfft_destroy_ragged": (None, [vp]),
                 "mfft_cos_sin_2pi": (None, [ctypes.c_uint64, ctypes.c_uint64, dp, dp]),
@@ -600,10 +602,13 @@ def ifft_unnormalized(x):
     return fft(x, BACKWARD)
 
 
-def ragged_queue(sizes, shapes):
-    """Host only: the two work queues of a ragged batch (mfft_ragged_queue). `shapes` lists
-    (n, stages, P, Q, G_A, G_B) per class in ascending n. Returns (tiles, n_first, offs) with
-    tiles an (ntiles, 5) int64 array of (class, stage, count, base, sub0)."""
+def ragged_queue(sizes, shapes, nqueue=1):
+    """Host only: the work queues of a ragged batch's two launches (mfft_ragged_queue).
+    `shapes` lists (n, stages, P, Q, G_A, G_B) per class in ascending n. Returns
+    (tiles, n_first, qoff, offs): tiles an (ntiles, 5) int64 array of
+    (class, stage, count, base, sub0), the first n_first of them the first launch's; queue q
+    of the first launch is tiles[qoff[q]:qoff[q + 1]], of the second
+    tiles[qoff[nqueue + 1 + q]:qoff[nqueue + 2 + q]]."""
     import numpy as np
 
     sz = np.ascontiguousarray(sizes, dtype=np.int64)
@@ -611,15 +616,16 @@ def ragged_queue(sizes, shapes):
     p64 = ctypes.POINTER(ctypes.c_int64)
     nf = ctypes.c_int64()
     n = lib().mfft_ragged_queue(len(sz), sz.ctypes.data_as(p64), len(sh), sh.ctypes.data_as(p64),
-                                None, 0, ctypes.byref(nf), None, 0)
+                                nqueue, None, 0, ctypes.byref(nf), None, None, 0)
     if n < 0:
         raise MfftError("inconsistent ragged queue description")
     tiles = np.zeros((max(n, 1), 5), dtype=np.int64)
+    qoff = np.zeros(2 * (nqueue + 1), dtype=np.int64)
     offs = np.zeros(max(len(sz), 1), dtype=np.int64)
     lib().mfft_ragged_queue(len(sz), sz.ctypes.data_as(p64), len(sh), sh.ctypes.data_as(p64),
-                            tiles.ctypes.data_as(p64), n, ctypes.byref(nf),
-                            offs.ctypes.data_as(p64), len(sz))
-    return tiles[:n], int(nf.value), offs
+                            nqueue, tiles.ctypes.data_as(p64), n, ctypes.byref(nf),
+                            qoff.ctypes.data_as(p64), offs.ctypes.data_as(p64), len(sz))
+    return tiles[:n], int(nf.value), qoff, offs
 
 
 from . import workloads  # noqa: E402,F401

@@ -3,12 +3,12 @@
 //
 // The host groups the transforms by size ("classes"; one plan per class, each plan the
 // same Stockham schedule the uniform path uses) and cuts the work into tiles of at most
-// LDS_ELEMS elements:
-//   * a single-stage class (N <= LDS_ELEMS): a tile is G = LDS_ELEMS / N transforms of
-//     that size, listed by their element offsets;
-//   * a two-stage class (N = P * Q): stage-A tiles of G_A columns, stage-B tiles of G_B
-//     residue classes, as in the uniform path; stage A writes its intermediate into the
-//     output buffer and stage B transforms it there in place.
+// 8 * RAGGED_THREADS = 4096 elements:
+//   * a single-stage class (N <= LDS_ELEMS): a tile is G = 4096 / N transforms of that size,
+//     listed by their element offsets;
+//   * a two-stage class (N = P * Q): stage-A tiles of G_A = 4096 / P columns, stage-B tiles
+//     of G_B = 4096 / Q residue classes; stage A writes its intermediate into the output
+//     buffer and stage B transforms it there in place.
 // Two queues, each largest class first (longest-processing-time order): the first holds
 // every stage-A tile and then every single-stage tile, the second every stage-B tile. One
 // persistent launch drains each queue (workgroups take tiles with an agent-scope atomic
@@ -28,6 +28,18 @@
 
 namespace mfft {
 
+// Elements per thread of a ragged tile: a workgroup of NT threads holds G * L <= 8 NT points.
+constexpr int RAGGED_EPT = 8;
+// Workgroup size of both launches. MI355X, the 10,000-transform batch: 256 threads (tiles of
+// 2048 points) 1.44-1.49 ms, 512 1.26-1.31 ms, 1024 1.37 ms; the longer rows of the wider
+// tiles leave fewer partly used 128-byte lines (HBM traffic 7.4 -> 6.5 GB at 512).
+constexpr int RAGGED_THREADS = 512;
+// Work queues per launch. One queue shared by every workgroup: with eight (one per XCD, each
+// size class's tiles in one queue, workgroup w draining queue w mod 8 first and then the
+// others) the HBM traffic of the 10,000-transform batch fell from 6.47 to 5.13 GB but both
+// launches ran about 40% longer on MI355X (1.26 -> 1.71 ms).
+constexpr int RAGGED_QUEUES = 1;
+
 struct RTile {
   int32_t cls;
   int32_t stage;  // 0: single stage or stage A, 1: stage B
@@ -37,21 +49,41 @@ struct RTile {
   int64_t sub0;   // two stage: first column / residue
 };
 
-template <bool FWD>
-__global__ __launch_bounds__(WG_THREADS) void ragged_kernel(
+// The queues of one launch: queue q holds tiles[off[q] .. off[q + 1]).
+struct RQueues {
+  int off[RAGGED_QUEUES + 1];
+};
+
+// Workgroup w drains queue w mod RAGGED_QUEUES first, then takes what is left of the others
+// in turn. Only thread 0 touches the queue heads; the tile index reaches the workgroup
+// through LDS.
+template <bool FWD, int NT>
+__global__ __launch_bounds__(NT) void ragged_kernel(
     const double2 *in, double2 *out, const double2 *__restrict__ tw,
-    const RClass *__restrict__ classes, const RTile *__restrict__ tiles, int ntiles,
-    const int64_t *__restrict__ offs, unsigned *head) {
+    const RClass *__restrict__ classes, const RTile *__restrict__ tiles, const RQueues qs,
+    const int64_t *__restrict__ offs, unsigned *heads) {
   extern __shared__ double2 lds[];
   __shared__ int s_tile;
   const int tid = threadIdx.x;
+  int qi = 0;  // thread 0: queues before the (blockIdx.x + qi)-th are known to be empty
   for (;;) {
-    if (tid == 0)
-      s_tile = (int)__hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) {
+      int found = -1;
+      for (; qi < RAGGED_QUEUES && found < 0; ++qi) {
+        const int q = (int)((blockIdx.x + qi) % RAGGED_QUEUES);
+        const unsigned k =
+            __hip_atomic_fetch_add(heads + q, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (k < (unsigned)(qs.off[q + 1] - qs.off[q])) {
+          found = qs.off[q] + (int)k;
+          break;  // this queue may hold more: try it again next time
+        }
+      }
+      s_tile = found;
+    }
     __syncthreads();
     const int it = s_tile;
     __syncthreads();
-    if (it >= ntiles) break;
+    if (it < 0) break;
     const RTile t = tiles[it];
     const RClass &c = classes[t.cls];
     const int st = t.stage;
@@ -77,12 +109,17 @@ __global__ __launch_bounds__(WG_THREADS) void ragged_kernel(
     // one pass body per radix with its HBM / LDS sides chosen at run time: 50 KiB of code and
     // 117 VGPRs (four waves per SIMD) against 133 KiB and 129 (three) with a copy per side
     // combination; MI355X 1.51 -> 1.58 TB/s on the 10,000-transform batch
-    run_stage_compact<FWD, RSET_MID, LDS_ELEMS / WG_THREADS>(lds, hin, hout, tid, WG_THREADS,
+    run_stage_compact<FWD, RSET_MID, RAGGED_EPT>(lds, hin, hout, tid, NT,
                                                             c.G[st], gcount, L, c.npass[st],
                                                             c.pass[st], gin, gout, tw, t.sub0,
                                                             st == 1 ? 1 : 0);
     __syncthreads();
   }
+}
+
+const void *ragged_kernel_for(bool fwd) {
+  return fwd ? (const void *)ragged_kernel<true, RAGGED_THREADS>
+             : (const void *)ragged_kernel<false, RAGGED_THREADS>;
 }
 
 __global__ void ragged_reset(unsigned *counters, int n) {
@@ -98,13 +135,14 @@ struct mfft_ragged {
   int sign = -1;
   int64_t count = 0, total = 0;
   int n1 = 0, n2 = 0;  // tiles of the first launch (stage A, single stage) and of the second
+  mfft::RQueues q1{}, q2{};  // their queues
   int grid = 0;
   size_t lds_bytes = 0;
   mfft::RClass *d_classes = nullptr;
   mfft::RTile *d_tiles1 = nullptr;
   mfft::RTile *d_tiles2 = nullptr;
   int64_t *d_offs = nullptr;
-  unsigned *d_counters = nullptr;  // the two queue heads
+  unsigned *d_counters = nullptr;  // the queue heads: the first launch's, then the second's
   double2 *d_tw = nullptr;
   std::vector<mfft_plan *> plans;  // one per class (twiddle layout source)
   std::map<int64_t, std::pair<std::vector<int>, int>> class_plan;  // size -> (radices, split)
@@ -120,14 +158,39 @@ struct RShape {
   int G0, G1;    // sub-problems per tile of each stage
 };
 
-// The two work queues of a ragged batch. `cls` lists the distinct sizes in ascending order
-// (class i has size cls[i].n). Transform i starts at element sum_{j<i} sizes[j]. First queue:
-// the stage-A tiles of the two-stage classes, then the single-stage tiles, each largest class
-// first; second queue: the stage-B tiles, largest class first. Single-stage tiles list their
-// transforms' offsets in `offs`. Returns false if a size has no class.
-bool ragged_queues(int64_t count, const int64_t *sizes, const std::vector<RShape> &cls,
+// Longest-processing-time assignment of weighted items to nq queues: heaviest first (the
+// items arrive largest class first, and the sort is stable), each to the least loaded queue
+// (the lowest index on ties).
+static std::vector<int> lpt_queues(const std::vector<int64_t> &weight, int nq) {
+  std::vector<int> order(weight.size()), q(weight.size(), 0);
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return weight[a] > weight[b]; });
+  std::vector<int64_t> load(nq, 0);
+  for (int i : order) {
+    const int best = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    q[i] = best;
+    load[best] += weight[i];
+  }
+  return q;
+}
+
+// The work queues of a ragged batch's two launches. `cls` lists the distinct sizes in
+// ascending order (class i has size cls[i].n). Transform i starts at element
+// sum_{j<i} sizes[j]. Each launch has nq queues (one per XCD), and every class's tiles of a
+// launch go to one queue, chosen by longest-processing-time on the class's points (so that
+// neighbouring tiles of a transform, which share partly used 128-byte lines, and the
+// transforms of a class, which share a twiddle table, meet in one XCD's L2). First launch,
+// queue q: the stage-A tiles of its two-stage classes, then its single-stage tiles, each
+// largest class first, a transform's tiles in sub-problem order; second launch: the stage-B
+// tiles, largest class first. `first` and `second` hold the queues back to back, queue q at
+// [qfirst[q], qfirst[q + 1]). Single-stage tiles list their transforms' offsets in `offs`.
+// Returns false if a size has no class.
+bool ragged_queues(int64_t count, const int64_t *sizes, const std::vector<RShape> &cls, int nq,
                    std::vector<RTile> &first, std::vector<RTile> &second,
+                   std::vector<int> &qfirst, std::vector<int> &qsecond,
                    std::vector<int64_t> &offs) {
+  if (nq < 1) return false;
   std::map<int64_t, std::vector<int64_t>> by_size;  // size -> transform offsets
   std::map<int64_t, int> cls_of;
   for (size_t i = 0; i < cls.size(); ++i) cls_of[cls[i].n] = (int)i;
@@ -137,14 +200,19 @@ bool ragged_queues(int64_t count, const int64_t *sizes, const std::vector<RShape
     by_size[sizes[i]].push_back(off);
     off += sizes[i];
   }
-  first.clear();
-  second.clear();
   offs.clear();
-  std::vector<RTile> single;
+  // the tiles of every class present, largest class first
+  struct Work {
+    bool two;
+    int64_t points;
+    std::vector<RTile> a, b;  // stage A (or single stage), stage B
+  };
+  std::vector<Work> work;
   for (auto it = by_size.rbegin(); it != by_size.rend(); ++it) {
     const int ci = cls_of[it->first];
     const RShape &c = cls[ci];
     const auto &starts = it->second;
+    Work w{c.nstage == 2, c.n * (int64_t)starts.size(), {}, {}};
     if (c.nstage == 1) {
       for (size_t k = 0; k < starts.size(); k += (size_t)c.G0) {
         RTile t{};
@@ -153,32 +221,58 @@ bool ragged_queues(int64_t count, const int64_t *sizes, const std::vector<RShape
         t.count = (int)std::min<size_t>((size_t)c.G0, starts.size() - k);
         t.base = (int64_t)offs.size();
         offs.insert(offs.end(), starts.begin() + k, starts.begin() + k + t.count);
-        single.push_back(t);
+        w.a.push_back(t);
       }
-      continue;
+    } else {
+      for (int64_t start : starts) {
+        for (int64_t c0 = 0; c0 < c.Q; c0 += c.G0) {
+          RTile t{};
+          t.cls = ci;
+          t.stage = 0;
+          t.count = (int)std::min<int64_t>(c.G0, c.Q - c0);
+          t.base = start;
+          t.sub0 = c0;
+          w.a.push_back(t);
+        }
+        for (int64_t k0 = 0; k0 < c.P; k0 += c.G1) {
+          RTile t{};
+          t.cls = ci;
+          t.stage = 1;
+          t.count = (int)std::min<int64_t>(c.G1, c.P - k0);
+          t.base = start;
+          t.sub0 = k0;
+          w.b.push_back(t);
+        }
+      }
     }
-    for (int64_t start : starts) {
-      for (int64_t c0 = 0; c0 < c.Q; c0 += c.G0) {
-        RTile t{};
-        t.cls = ci;
-        t.stage = 0;
-        t.count = (int)std::min<int64_t>(c.G0, c.Q - c0);
-        t.base = start;
-        t.sub0 = c0;
-        first.push_back(t);
-      }
-      for (int64_t k0 = 0; k0 < c.P; k0 += c.G1) {
-        RTile t{};
-        t.cls = ci;
-        t.stage = 1;
-        t.count = (int)std::min<int64_t>(c.G1, c.P - k0);
-        t.base = start;
-        t.sub0 = k0;
-        second.push_back(t);
-      }
+    work.push_back(std::move(w));
+  }
+  std::vector<int64_t> wa(work.size()), wb;
+  std::vector<int> two_idx;
+  for (size_t i = 0; i < work.size(); ++i) {
+    wa[i] = work[i].points;
+    if (work[i].two) {
+      two_idx.push_back((int)i);
+      wb.push_back(work[i].points);
     }
   }
-  first.insert(first.end(), single.begin(), single.end());
+  const std::vector<int> qa = lpt_queues(wa, nq), qb = lpt_queues(wb, nq);
+  first.clear();
+  second.clear();
+  qfirst.assign(nq + 1, 0);
+  qsecond.assign(nq + 1, 0);
+  for (int q = 0; q < nq; ++q) {
+    qfirst[q] = (int)first.size();
+    for (int pass = 0; pass < 2; ++pass)  // two-stage classes first, then single-stage ones
+      for (size_t i = 0; i < work.size(); ++i)
+        if (qa[i] == q && work[i].two == (pass == 0))
+          first.insert(first.end(), work[i].a.begin(), work[i].a.end());
+    qsecond[q] = (int)second.size();
+    for (size_t j = 0; j < two_idx.size(); ++j)
+      if (qb[j] == q) second.insert(second.end(), work[two_idx[j]].b.begin(), work[two_idx[j]].b.end());
+  }
+  qfirst[nq] = (int)first.size();
+  qsecond[nq] = (int)second.size();
   return true;
 }
 
@@ -217,6 +311,8 @@ mfft_ragged *mfft_plan_ragged(int64_t count, const int64_t *sizes, int sign, uns
     r->total += sizes[i];
   }
 
+  const int tile = mfft::RAGGED_EPT * mfft::RAGGED_THREADS;  // points per tile
+
   std::vector<mfft::RClass> classes;
   std::vector<mfft::RShape> shapes;
   std::vector<double2> tw;
@@ -238,17 +334,33 @@ mfft_ragged *mfft_plan_ragged(int64_t count, const int64_t *sizes, int sign, uns
       free_ragged(r);
       return nullptr;
     }
+    // sub-problems per tile: as many as the tile holds (at most the stage's count)
+    for (int s2 = 0; s2 < c.nstage; ++s2) {
+      int64_t g = std::max(1, tile / c.L[s2]);
+      if (c.nstage == 2) g = std::min<int64_t>(g, s2 == 0 ? c.Q : c.P);
+      while (g > 1 && (int64_t)mfft::stage_lds_elems((int)g, c.L[s2], c.npass[s2]) *
+                              (int64_t)sizeof(double2) > mfft::kLdsBytesMax)
+        --g;
+      c.G[s2] = (int)g;
+      for (int q = 0; q < c.npass[s2]; ++q) c.pass[s2][q].m_G = mfft::magic32(c.G[s2]);
+    }
     classes.push_back(c);
     shapes.push_back({kv.first, c.nstage, c.P, c.Q, c.G[0], c.nstage == 2 ? c.G[1] : 0});
   }
   std::vector<mfft::RTile> first, second;
   std::vector<int64_t> tile_offs;
-  if (!mfft::ragged_queues(count, sizes, shapes, first, second, tile_offs)) {
+  std::vector<int> qf, qs;
+  if (!mfft::ragged_queues(count, sizes, shapes, mfft::RAGGED_QUEUES, first, second, qf, qs,
+                           tile_offs)) {
     free_ragged(r);
     return nullptr;
   }
   r->n1 = (int)first.size();
   r->n2 = (int)second.size();
+  for (int q = 0; q <= mfft::RAGGED_QUEUES; ++q) {
+    r->q1.off[q] = qf[q];
+    r->q2.off[q] = qs[q];
+  }
 
   auto up = [](void **dst, const void *src, size_t bytes) {
     if (bytes == 0) bytes = 16;
@@ -260,7 +372,7 @@ mfft_ragged *mfft_plan_ragged(int64_t count, const int64_t *sizes, int sign, uns
             up((void **)&r->d_tiles2, second.data(), second.size() * sizeof(mfft::RTile)) &&
             up((void **)&r->d_offs, tile_offs.empty() ? nullptr : tile_offs.data(),
                tile_offs.size() * sizeof(int64_t)) &&
-            up((void **)&r->d_counters, nullptr, 2 * sizeof(unsigned)) &&
+            up((void **)&r->d_counters, nullptr, 2 * mfft::RAGGED_QUEUES * sizeof(unsigned)) &&
             up((void **)&r->d_tw, tw.empty() ? nullptr : tw.data(), tw.size() * sizeof(double2));
   if (!ok) {
     free_ragged(r);
@@ -277,12 +389,12 @@ mfft_ragged *mfft_plan_ragged(int64_t count, const int64_t *sizes, int sign, uns
       need_lds = std::max(need_lds, (size_t)mfft::stage_lds_elems(c.G[s2], c.L[s2], c.npass[s2]) *
                                         sizeof(double2));
   r->lds_bytes = need_lds;
-  for (const void *k : {(const void *)mfft::ragged_kernel<true>,
-                        (const void *)mfft::ragged_kernel<false>})
-    (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, mfft::kLdsBytes);
-  // a few resident workgroups per CU (about 35 KiB of LDS each); workgroups that do not fit
-  // start when others finish and take what is left of the queue
-  r->grid = std::max(1, std::max(ncu, 1) * 4);
+  for (bool fwd : {true, false})
+    (void)hipFuncSetAttribute(mfft::ragged_kernel_for(fwd),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)need_lds);
+  // two resident workgroups per CU (tiles of up to 70 KiB of LDS); workgroups that do not
+  // fit start when others finish and take what is left of the queue
+  r->grid = std::max(1, std::max(ncu, 1) * (1024 / mfft::RAGGED_THREADS));
   return r;
 }
 
@@ -291,9 +403,9 @@ int mfft_execute_ragged(const mfft_ragged *r, const void *in, void *out, void *s
   if (in == out) return MFFT_ERR_UNSUPPORTED;  // stage A writes its intermediate into out
   if (r->n1 + r->n2 == 0) return MFFT_SUCCESS;
   hipStream_t st = (hipStream_t)stream;
-  unsigned *ctr = r->d_counters;  // [0] first queue's head, [1] second queue's head
+  unsigned *ctr = r->d_counters;  // the first launch's queue heads, then the second's
   {
-    int nc = 2;
+    int nc = 2 * mfft::RAGGED_QUEUES;
     void *args[] = {(void *)&ctr, (void *)&nc};
     if (hipLaunchKernel((const void *)mfft::ragged_reset, dim3(1), dim3(256), args, 0, st) !=
         hipSuccess)
@@ -301,17 +413,16 @@ int mfft_execute_ragged(const mfft_ragged *r, const void *in, void *out, void *s
   }
   const double2 *x = (const double2 *)in;
   double2 *y = (double2 *)out;
-  const void *k = r->sign < 0 ? (const void *)mfft::ragged_kernel<true>
-                              : (const void *)mfft::ragged_kernel<false>;
-  auto launch = [&](const mfft::RTile *tiles, int n, unsigned *head) {
+  const void *k = mfft::ragged_kernel_for(r->sign < 0);
+  auto launch = [&](const mfft::RTile *tiles, int n, const mfft::RQueues &qs, unsigned *heads) {
     if (n == 0) return hipSuccess;
     void *args[] = {(void *)&x, (void *)&y, (void *)&r->d_tw, (void *)&r->d_classes,
-                    (void *)&tiles, (void *)&n, (void *)&r->d_offs, (void *)&head};
-    return hipLaunchKernel(k, dim3(std::max(1, std::min(n, r->grid))), dim3(mfft::WG_THREADS),
+                    (void *)&tiles, (void *)&qs, (void *)&r->d_offs, (void *)&heads};
+    return hipLaunchKernel(k, dim3(std::max(1, std::min(n, r->grid))), dim3(mfft::RAGGED_THREADS),
                            args, r->lds_bytes, st);
   };
-  if (launch(r->d_tiles1, r->n1, ctr) != hipSuccess ||
-      launch(r->d_tiles2, r->n2, ctr + 1) != hipSuccess)
+  if (launch(r->d_tiles1, r->n1, r->q1, ctr) != hipSuccess ||
+      launch(r->d_tiles2, r->n2, r->q2, ctr + mfft::RAGGED_QUEUES) != hipSuccess)
     return MFFT_ERR_HIP;
   return MFFT_SUCCESS;
 }
@@ -333,9 +444,12 @@ int mfft_ragged_class_plan(const mfft_ragged *r, int64_t n, int *radices, int ma
 }
 
 int64_t mfft_ragged_queue(int64_t count, const int64_t *sizes, int64_t nclass,
-                          const int64_t *shapes, int64_t *tiles, int64_t max_tiles,
-                          int64_t *n_first, int64_t *offs, int64_t max_offs) {
-  if (count < 0 || nclass < 0 || (count && !sizes) || (nclass && !shapes)) return -1;
+                          const int64_t *shapes, int64_t nqueue, int64_t *tiles,
+                          int64_t max_tiles, int64_t *n_first, int64_t *qoff, int64_t *offs,
+                          int64_t max_offs) {
+  if (count < 0 || nclass < 0 || (count && !sizes) || (nclass && !shapes) || nqueue < 1 ||
+      nqueue > 64)
+    return -1;
   std::vector<mfft::RShape> cls;
   for (int64_t i = 0; i < nclass; ++i) {
     const int64_t *s = shapes + 6 * i;
@@ -346,8 +460,14 @@ int64_t mfft_ragged_queue(int64_t count, const int64_t *sizes, int64_t nclass,
   }
   std::vector<mfft::RTile> first, second;
   std::vector<int64_t> o;
-  if (!mfft::ragged_queues(count, sizes, cls, first, second, o)) return -1;
+  std::vector<int> qf, qs;
+  if (!mfft::ragged_queues(count, sizes, cls, (int)nqueue, first, second, qf, qs, o)) return -1;
   if (n_first) *n_first = (int64_t)first.size();
+  if (qoff)
+    for (int64_t q = 0; q <= nqueue; ++q) {
+      qoff[q] = qf[q];
+      qoff[nqueue + 1 + q] = (int64_t)first.size() + qs[q];
+    }
   first.insert(first.end(), second.begin(), second.end());
   for (size_t i = 0; i < first.size() && (int64_t)i < max_tiles && tiles; ++i) {
     const mfft::RTile &t = first[i];

@@ -202,8 +202,11 @@ def test_ragged_queue_covers_every_tile_once(lf):
     """The host queue builder of ragged batches (mfft_ragged_queue, host only): every
     transform's tiles are queued exactly once, every stage-B tile in the second launch and
     every stage-A / single-stage tile in the first (so the launch boundary orders each
-    transform's stages), the stage-A tiles ahead of the single-stage ones, classes largest
-    first, tiles within their sub-problem range, single-stage tiles listing the offsets of
+    transform's stages); each launch has `nqueue` queues, a class's tiles of a launch all in
+    one queue, the queues' points balanced by longest-processing-time (no two queues differ by
+    more than the heaviest class); within a queue the stage-A tiles come ahead of the
+    single-stage ones, classes largest first, a transform's tiles in sub-problem order; tiles
+    stay within their sub-problem range, and single-stage tiles list the offsets of
     transforms of their own size."""
     import numpy as np
 
@@ -212,17 +215,35 @@ def test_ragged_queue_covers_every_tile_once(lf):
               (4096, 2, 64, 64, 16, 8), (10368, 2, 96, 108, 21, 12), (65520, 2, 240, 273, 8, 8)]
     rng = np.random.default_rng(3)
     sizes = rng.choice([s[0] for s in shapes], size=400)
-    tiles, n_first, offs = lf.ragged_queue(sizes, shapes)
+    counts = {n: int((sizes == n).sum()) for n in set(sizes.tolist())}
+    for nq in (1, 3, 8):
+        tiles, n_first, qoff, offs = lf.ragged_queue(sizes, shapes, nq)
+        check_ragged_queues(shapes, sizes, counts, tiles, n_first, qoff, offs, nq)
+
+
+def check_ragged_queues(shapes, sizes, counts, tiles, n_first, qoff, offs, nq):
+    import numpy as np
+
     starts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
     first, second = tiles[:n_first], tiles[n_first:]
     assert (second[:, 1] == 1).all() and (first[:, 1] == 0).all()
+    assert qoff[0] == 0 and qoff[nq] == n_first and qoff[nq + 1] == n_first
+    assert qoff[2 * nq + 1] == len(tiles) and (np.diff(qoff) >= 0).all()
     two = [i for i, s in enumerate(shapes) if s[1] == 2]
-    is_two = np.isin(first[:, 0], two)
-    # stage-A tiles first, then single-stage tiles
-    assert not (np.diff(is_two.astype(int)) > 0).any()
-    # largest class first within each run
-    for part in (first[is_two], first[~is_two], second):
-        assert (np.diff(part[:, 0]) <= 0).all()
+    for launch in (0, 1):
+        home, load = {}, np.zeros(nq)
+        for q in range(nq):
+            part = tiles[qoff[launch * (nq + 1) + q]:qoff[launch * (nq + 1) + q + 1]]
+            is_two = np.isin(part[:, 0], two)
+            # stage-A tiles first, then single-stage tiles; largest class first within each
+            assert not (np.diff(is_two.astype(int)) > 0).any()
+            for run in (part[is_two], part[~is_two]):
+                assert (np.diff(run[:, 0]) <= 0).all()
+            for cls in set(part[:, 0].tolist()):
+                assert home.setdefault(cls, q) == q  # a class's tiles in one queue
+                load[q] += shapes[cls][0] * counts[shapes[cls][0]]
+        heaviest = max(shapes[c][0] * counts[shapes[c][0]] for c in home) if home else 0
+        assert load.max() - load.min() <= heaviest
     seen_single = []
     cover = {}
     for cls, stage, count, base, sub0 in tiles:
@@ -244,14 +265,21 @@ def test_ragged_queue_covers_every_tile_once(lf):
         if ns != 2:
             continue
         for stage, lim in ((0, Q), (1, P)):
-            runs = sorted(cover[(int(o), stage)])
+            runs = cover[(int(o), stage)]
+            assert [a for a, _ in runs] == sorted(a for a, _ in runs)  # sub-problem order
             pos = 0
             for a, c in runs:  # contiguous, no overlap, complete
                 assert a == pos
                 pos += c
             assert pos == lim
+
+
+def test_ragged_queue_rejects_bad_input(lf):
+    shapes = [(6, 1, 0, 0, 341, 0), (60, 1, 0, 0, 34, 0)]
     with pytest.raises(lf.MfftError):
         lf.ragged_queue([6, 7], shapes)  # a size without a class
+    with pytest.raises(lf.MfftError):
+        lf.ragged_queue([6, 60], shapes, nqueue=0)
 
 
 def shapes_n_stages(shapes, n):
