@@ -1544,7 +1544,7 @@ int64_t mfft::workspace_elems(const mfft_plan *p) {
 int mfft::export_class(const mfft_plan *p, mfft::RClass *c, std::vector<double2> *tw) {
   if (!p || p->howmany != 1 || p->stages.empty() || p->stages.size() > 2) return 1;
   for (const auto &st : p->stages)
-    if (st.ept != 8) return 1;  // the ragged kernel runs the 256-thread layout only
+    if (st.ept != 8) return 1;  // the ragged kernel runs 8 elements per thread only
   if (!all_mid(p->radices)) return 1;  // and holds the mid radix set
   const int64_t base = (int64_t)tw->size();
   *c = mfft::RClass{};
