@@ -112,9 +112,11 @@ def libbench2_time(run, min_seconds=0.01, samples=8):
 
 def cpu_baseline_sweep(sizes, plans_radices, budget_s):
     """The oracle -- the C restatement of the reference's refTransform with the GPU plans'
-    radices -- on one pinned core, timed per size with libbench2's rule on one transform
-    repeated in C (as libbench2 drives the reference), and scaled to the sweep's batch of
-    2^24 / N transforms per size. The reference's own published per-size times (another
+    radices -- on one pinned core, timed per size with libbench2's rule on a call repeated in
+    C (as libbench2 drives the reference) and scaled to the sweep's batch of 2^24 / N
+    transforms per size. Each call transforms four inputs at once, one per lane of a
+    four-double AVX2 vector (the scalar oracle's operations lane by lane, so its bits; 1.0 to
+    3.9 times the scalar path's rate). The reference's own published per-size times (another
     host) are given beside it, aggregated the same way."""
     sys.path.insert(0, ROOT)
     import numpy as np
@@ -130,9 +132,9 @@ def cpu_baseline_sweep(sizes, plans_radices, budget_s):
     try:
         for N in sizes:
             rp = oracle.RefPlan(True, N, plans_radices[N])
-            x = oracle.lcg_input(N, seed=oracle.LCG_SEED ^ N)
+            x = np.ascontiguousarray(oracle.lcg_input(N, seed=oracle.LCG_SEED ^ N, howmany=4))
             out = np.empty_like(x)
-            t = libbench2_time(lambda reps: rp.repeat(x, out, reps))
+            t = libbench2_time(lambda reps: rp.repeat_x4(x, out, reps)) / 4
             batch = (1 << 24) // N
             total_t += t * batch
             total_b += 32.0 * N * batch
@@ -149,7 +151,8 @@ def cpu_baseline_sweep(sizes, plans_radices, budget_s):
         "cores": 1,
         "kind": "port",
         "sample": f"oracle refTransform (C: gcc {' '.join(oracle.CFLAGS)}) with the GPU plans' "
-                  f"radices, one transform per size for {done} of the {len(sizes)} sweep sizes, "
+                  f"radices, four transforms per call (one per lane of an AVX2 vector; the "
+                  f"scalar oracle's bits), per size for {done} of the {len(sizes)} sweep sizes, "
                   f"pinned to core {core}, libbench2's rule (repetitions doubled to >= 10 ms, "
                   f"min of 8), scaled to batch 2^24/N; host {host_cpu_model()}, "
                   f"{os.cpu_count()} logical CPUs",

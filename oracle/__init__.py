@@ -49,12 +49,14 @@ def build(force: bool = False) -> str:
 
     os.makedirs(BUILD, exist_ok=True)
     gen = os.path.join(BUILD, "bfly_gen.c")
-    src = cgen.emit_all()
-    old = open(gen).read() if os.path.exists(gen) else None
-    if old != src:
-        with open(gen, "w") as f:
-            f.write(src)
-    srcs = [gen, os.path.join(HERE, "refstockham.c"), os.path.join(HERE, "mpfft.c")]
+    vgen = os.path.join(BUILD, "vbfly_gen.c")
+    for path, src in ((gen, cgen.emit_all()), (vgen, cgen.emit_vector_all())):
+        old = open(path).read() if os.path.exists(path) else None
+        if old != src:
+            with open(path, "w") as f:
+                f.write(src)
+    srcs = [gen, vgen, os.path.join(HERE, "refstockham.c"), os.path.join(HERE, "vstockham.c"),
+            os.path.join(HERE, "mpfft.c")]
     if not force and os.path.exists(LIB):
         mt = os.path.getmtime(LIB)
         if all(os.path.getmtime(s) <= mt for s in srcs):
@@ -84,6 +86,13 @@ def lib():
             L.oracle_ref_transform.restype = ctypes.c_int
             L.oracle_ref_repeat.argtypes = L.oracle_ref_transform.argtypes
             L.oracle_ref_repeat.restype = ctypes.c_int
+            L.oracle_ref_transform_x4.argtypes = [
+                ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_int), dp,
+                dp, dp, dp,
+            ]
+            L.oracle_ref_transform_x4.restype = ctypes.c_int
+            L.oracle_ref_repeat_x4.argtypes = L.oracle_ref_transform.argtypes
+            L.oracle_ref_repeat_x4.restype = ctypes.c_int
             L.oracle_butterfly.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
             L.oracle_butterfly.restype = ctypes.c_int
             L.oracle_mp_dft.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, dp, dp, dp]
@@ -290,6 +299,35 @@ class RefPlan:
             1 if self.fwd else 0, self.N, len(self.rs), self.rsa, _dp(self.tw), int(reps),
             _dp(x.reshape(-1).view(np.float64)), _dp(out.reshape(-1).view(np.float64)),
             _dp(self.work),
+        )
+        if rc != 0:
+            raise ValueError("oracle cannot run this plan")
+
+    def _work_x4(self) -> np.ndarray:
+        """24 N doubles on a 32-byte boundary (the four-lane path's vectors)."""
+        if getattr(self, "_w4", None) is None:
+            raw = np.empty(24 * self.N + 4)
+            off = (-raw.ctypes.data % 32) // 8
+            self._w4 = raw[off:off + 24 * self.N]
+        return self._w4
+
+    def execute_x4(self, x: np.ndarray, out: np.ndarray) -> None:
+        """The four-lane path (one transform per lane of a four-double vector): x, out
+        complex128 of shape (4, N), C-contiguous; the scalar path's bits."""
+        rc = lib().oracle_ref_transform_x4(
+            1 if self.fwd else 0, self.N, len(self.rs), self.rsa, _dp(self.tw),
+            _dp(x.reshape(-1).view(np.float64)), _dp(out.reshape(-1).view(np.float64)),
+            _dp(self._work_x4()),
+        )
+        if rc != 0:
+            raise ValueError("oracle cannot run this plan")
+
+    def repeat_x4(self, x: np.ndarray, out: np.ndarray, reps: int) -> None:
+        """The four-transform call `reps` times on the same x (shape (4, N)) into out."""
+        rc = lib().oracle_ref_repeat_x4(
+            1 if self.fwd else 0, self.N, len(self.rs), self.rsa, _dp(self.tw), int(reps),
+            _dp(x.reshape(-1).view(np.float64)), _dp(out.reshape(-1).view(np.float64)),
+            _dp(self._work_x4()),
         )
         if rc != 0:
             raise ValueError("oracle cannot run this plan")

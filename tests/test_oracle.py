@@ -416,3 +416,26 @@ def test_composite_maps_vs_numpy(oracle):
     b = oracle.ref_transform(True, [16], np.ascontiguousarray(a).reshape(-1, n1)).reshape(x.shape)
     ref = np.fft.fft2(x)
     assert np.linalg.norm(b - ref) / np.linalg.norm(ref) < 1e-14
+
+
+def test_four_lane_path_matches_scalar(oracle):
+    """The CPU baseline's four-lane path (four transforms per call, one per lane of a
+    four-double vector, each butterfly's operations applied lane by lane) returns the scalar
+    oracle's bits: every radix the plans use (2..16 incl. 11 and 13), twiddled and untwiddled
+    passes, both directions, one- to seven-pass plans."""
+    rng = np.random.default_rng(11)
+    plans = [(2, [2]), (6, [6]), (143, [11, 13]), (1024, [16, 8, 8]), (2048, [16, 8, 16]),
+             (4096, [8, 8, 8, 8]), (10368, [12, 8, 12, 9]), (2100, [15, 14, 10]),
+             (75600, [10, 9, 3, 10, 7, 4]), (165375, [5, 5, 5, 3, 9, 7, 7])]
+    for N, rs in plans:
+        for fwd in (True, False):
+            x = (rng.random((4, N)) - 0.5) + 1j * (rng.random((4, N)) - 0.5)
+            rp = oracle.RefPlan(fwd, N, rs)
+            a = np.empty_like(x)
+            rp.execute(x, a)
+            b = np.full_like(x, np.nan)
+            rp.execute_x4(x, b)
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (N, rs, fwd)
+            c = np.full_like(x, np.nan)
+            rp.repeat_x4(x, c, 2)
+            assert np.array_equal(a.view(np.uint64), c.view(np.uint64)), (N, rs, fwd)
