@@ -606,9 +606,12 @@ bool build_stages(mfft_plan *p, int split, int split2 = 0) {
       a.threads = T < 256 ? 256 : T;
       a.G = a.threads / T;
       a.stage_io = 0;
-      // ordinary stores of the output measured 2-8% faster than non-temporal ones for these
-      // kernels on MI355X (64..8192 points; neutral at 16 elements per thread, 16384)
-      a.cached_out = E == 8 ? 1 : 0;
+      // at 8 elements per thread, ordinary loads and non-temporal stores (MI355X, 4096 points
+      // per 256 MB batch: 89.8 us with non-temporal loads and ordinary stores, 102.6 with both
+      // ordinary, 82.2 with this); at 16 (8192, 16384 points) both non-temporal (ordinary
+      // loads within +-2%)
+      a.cached_in = E == 8 ? 1 : 0;
+      a.cached_out = 0;
     } else if (a.reg == 0 && !a.gfast_in && !a.stage_io) {
       // the LDS-tile kernel's first pass reads each butterfly's inputs in runs of a few
       // elements, so several load instructions share a cache line: ordinary loads keep it
@@ -752,6 +755,10 @@ int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
     a.pre_k = pre_k;
     a.post = post;
     a.sumsq = sumsq;
+    // a call with input weights is the adjoint of the differentiable path, whose input (the
+    // forward's output) was just written non-temporally: non-temporal loads and stores there
+    // (MI355X, (512, 4096) forward + backward 35.3 us with the plain policy, 33.3 with this)
+    if (a.reg == 1 && pre) a.cached_in = a.cached_out = 0;
     if (p->outer == 1)
       return mfft::launch_stage(fwd, a, 1, in, out, p->d_tw, st) == hipSuccess ? MFFT_SUCCESS
                                                                               : MFFT_ERR_HIP;
