@@ -179,6 +179,14 @@ hipError_t launch_tile2d(bool fwd, int idx, const StageArgs &col, const StageArg
                          const double2 *in, double2 *out, const double2 *tw0,
                          const double2 *tw1, int64_t howmany, hipStream_t stream);
 
+// Single-pass 2-D transforms of arrays that fill one workgroup's registers (mfft_split.hip:
+// 128 x 128): the kernel for an n0 x n1 plan whose parts are single stages of its radices
+// (nullptr if none), and its launch over `arrays` contiguous arrays.
+const void *split2d_lookup(int64_t n0, int64_t n1, const StageArgs &col, const StageArgs &row);
+hipError_t launch_split2d(bool fwd, const void *sched, const StageArgs &col, const StageArgs &row,
+                          const double2 *in, double2 *out, const double2 *tw0,
+                          const double2 *tw1, int64_t arrays, hipStream_t stream);
+
 // A caller-provided intermediate for two-stage plans: `elems` complex values at `ptr`.
 struct Workspace {
   double2 *ptr;

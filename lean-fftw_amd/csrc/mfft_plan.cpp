@@ -61,6 +61,7 @@ struct mfft_plan {
   bool fused_io = false;  // r2c / c2r: the real-data maps fuse into the complex part's kernels
   int64_t n0 = 0, n1 = 0;
   int fused2d = 0;  // 2-D: 1 + index of the single-pass kernel for small arrays, 0 for none
+  const void *split2d = nullptr;  // 2-D: the single-pass register kernel (128 x 128), if any
   // an outer batch of single-stage plans: `outer` repetitions of the whole batch, the k-th
   // at in + k * outer_in, out + k * outer_out (the arrays of a 2-D plan's axis-0 part)
   int64_t outer = 1, outer_in = 0, outer_out = 0;
@@ -918,6 +919,12 @@ int run_composite(const mfft_plan *p, const void *in, void *out, hipStream_t st,
                  : MFFT_ERR_HIP;
     }
     case 3: {  // 2-D: axis 0 (columns, stride n1) of every array, then axis 1 (rows)
+      if (p->split2d)  // arrays of 16384 points: both axes in one workgroup's registers
+        return mfft::launch_split2d(p->sign < 0, p->split2d, p->part[0]->stages[0],
+                                    p->part[1]->stages[0], (const double2 *)in, (double2 *)out,
+                                    p->part[0]->d_tw, p->part[1]->d_tw, hm, st) == hipSuccess
+                   ? MFFT_SUCCESS
+                   : MFFT_ERR_HIP;
       if (p->fused2d)  // small arrays: both axes in one pass through LDS
         return mfft::launch_tile2d(p->sign < 0, p->fused2d, p->part[0]->stages[0],
                                    p->part[1]->stages[0], (const double2 *)in, (double2 *)out,
@@ -1750,6 +1757,8 @@ mfft_plan *mfft_plan_dft_2d(int64_t n0, int64_t n1, int sign, int64_t howmany, u
   p->stage_passes = row->stage_passes;
   if (col->stages.size() == 1 && row->stages.size() == 1 && n0 * n1 <= 4096)
     p->fused2d = mfft::tile2d_lookup(n0, n1, col->stages[0], row->stages[0]);
+  if (col->stages.size() == 1 && row->stages.size() == 1)
+    p->split2d = mfft::split2d_lookup(n0, n1, col->stages[0], row->stages[0]);
   return p;
 }
 

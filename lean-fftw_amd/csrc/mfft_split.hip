@@ -272,6 +272,82 @@ __global__ __launch_bounds__(product(RA{}) * product(RB{}) / E, 4) void split_ke
                                                                   lds_split);
 }
 
+// ---------------------------------------------------------------- 2-D arrays
+// A row-major P x Q array (element (i0, i1) at i0 Q + i1) in one workgroup: the column phase
+// above is the axis-0 transform (column q = elements q + Q i0, the column plan's passes and
+// table), the hand-over leaves row k0 = column outputs k0, and the row phase runs the row
+// plan's passes and table on each row (no factor between the axes) and writes X[k0, k1] at
+// k0 Q + k1. The two-launch 2-D plan applies the same line transforms in the same order (axis
+// 0, then axis 1), so the bits are its bits. Row-phase lanes walk along the row
+// (lane l: thread l mod TR of row w RW + l / TR), so the stores move whole 128-byte lines.
+template <bool FWD, int E, int... As, int... Bs>
+__device__ __forceinline__ void split2d_body(Radices<As...>, Radices<Bs...>, const double2 *in,
+                                             double2 *out, const double2 *__restrict__ tw0,
+                                             const double2 *__restrict__ tw1,
+                                             const StageArgs &col, const StageArgs &row,
+                                             double *lds) {
+  constexpr int P = (1 * ... * As), Q = (1 * ... * Bs), N = P * Q;
+  constexpr int TC = P / E, TR = Q / E;
+  constexpr int CW = 64 / TC, RW = 64 / TR;
+  static_assert(Q >= 32, "the transpose swizzle permutes 32-element blocks of a row");
+  static_assert(TC <= 32 && TR <= 32 && 32 % TC == 0 && 32 % TR == 0, "columns and rows fit a wave");
+  const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+  const int64_t h = blockIdx.x;  // one array per workgroup
+  const int cw = lane % CW, tc = lane / CW, q = w * CW + cw;
+  double2 v[E];
+  split_load<Q, TC, E, false>(v, in + h * N, col, q, tc);
+  col_passes<FWD, TC, E, 0, 1, As...>(v, tc, lds + q * spad_len(P), col, tw0);
+  __syncthreads();
+  const int tr = lane % TR, k = w * RW + lane / TR;
+  const int sk = xsw(k);
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int o = tc + TC * m;
+    lds[o * Q + (q ^ xsw(o))] = v[m].x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m].x = lds[k * Q + ((tr + TR * m) ^ sk)];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int o = tc + TC * m;
+    lds[o * Q + (q ^ xsw(o))] = v[m].y;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m].y = lds[k * Q + ((tr + TR * m) ^ sk)];
+  __syncthreads();
+  col_passes<FWD, TR, E, 0, 1, Bs...>(v, tr, lds + k * spad_len(Q), row, tw1);
+  int trs = tr;
+  asm volatile("" : "+v"(trs));  // keep the store addresses out of the pass code
+  double2 *d = out + h * N + (int64_t)k * Q + trs;
+#pragma unroll
+  for (int m = 0; m < E; ++m) st_stream(d + TR * m, v[m]);
+}
+
+template <bool FWD, int E, class RA, class RB>
+__global__ __launch_bounds__(product(RA{}) * product(RB{}) / E, 4) void split2d_kernel(
+    const double2 *in, double2 *out, const double2 *__restrict__ tw0,
+    const double2 *__restrict__ tw1, const StageArgs col, const StageArgs row) {
+  extern __shared__ double lds_split[];
+  split2d_body<FWD, E>(RA{}, RB{}, in, out, tw0, tw1, col, row, lds_split);
+}
+
+struct Split2dSchedule {
+  int n0, n1, e, na, nb;
+  int ra[4], rb[4];
+  size_t lds;  // bytes
+  const void *k[2];
+};
+
+using R882 = Radices<8, 8, 2>;
+const Split2dSchedule kSplit2d[] = {
+    {128, 128, 16, 3, 3, {8, 8, 2}, {8, 8, 2}, (size_t)split_lds_doubles(R882{}, R882{}) * sizeof(double),
+     {(const void *)split2d_kernel<false, 16, R882, R882>,
+      (const void *)split2d_kernel<true, 16, R882, R882>}},
+};
+
 struct SplitSchedule {
   int n, e, na, nb;
   int rs[8];
@@ -309,6 +385,38 @@ const void *split_schedule(int64_t n, const int *radices, int nradix) {
     if (same) return &s;
   }
   return nullptr;
+}
+
+// The single-pass 2-D kernel for an n0 x n1 plan whose axis parts are single stages of these
+// radices (nullptr if there is none).
+const void *split2d_lookup(int64_t n0, int64_t n1, const StageArgs &col, const StageArgs &row) {
+  for (const auto &s : kSplit2d) {
+    if (s.n0 != n0 || s.n1 != n1 || s.na != col.npass || s.nb != row.npass) continue;
+    bool same = true;
+    for (int i = 0; i < s.na; ++i) same = same && s.ra[i] == col.pass[i].R;
+    for (int i = 0; i < s.nb; ++i) same = same && s.rb[i] == row.pass[i].R;
+    if (same) return &s;
+  }
+  return nullptr;
+}
+
+hipError_t launch_split2d(bool fwd, const void *sched, const StageArgs &col, const StageArgs &row,
+                          const double2 *in, double2 *out, const double2 *tw0,
+                          const double2 *tw1, int64_t arrays, hipStream_t stream) {
+  const auto *s = static_cast<const Split2dSchedule *>(sched);
+  if (!s || arrays < 1) return arrays < 1 ? hipSuccess : hipErrorInvalidValue;
+  const void *k = s->k[fwd ? 1 : 0];
+  {
+    static std::mutex mu;
+    static std::set<const void *> done;
+    std::lock_guard<std::mutex> lk(mu);
+    if (done.insert(k).second)
+      (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds);
+  }
+  void *args[] = {(void *)&in, (void *)&out, (void *)&tw0, (void *)&tw1, (void *)&col,
+                  (void *)&row};
+  return hipLaunchKernel(k, dim3((unsigned)arrays), dim3(s->n0 * s->n1 / s->e), args, s->lds,
+                         stream);
 }
 
 // The split kernel for a single-stage register plan of these radices (complex data, one

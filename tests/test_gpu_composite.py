@@ -140,8 +140,9 @@ def oracle_2d(lf, oracle, fwd, x):
     return oracle.ref_transform(fwd, rs1, rows).reshape(hm, n0, n1)
 
 
+# (128, 128, 5): the single-pass register kernel (both axes in one workgroup)
 SHAPES_2D = [(2, 2, 1), (3, 5, 2), (8, 8, 3), (16, 1000, 1), (64, 64, 4), (12, 4096, 1),
-             (256, 256, 2), (1000, 12, 1), (1024, 1024, 1)]
+             (128, 128, 5), (256, 256, 2), (1000, 12, 1), (1024, 1024, 1)]
 
 
 @pytest.mark.parametrize("n0,n1,hm", SHAPES_2D)
@@ -158,9 +159,9 @@ def test_2d_bit_exact(lf, oracle, n0, n1, hm, fwd):
     assert rel_l2(got, ref) < TOL
 
 
-def test_2d_in_place(lf, oracle):
-    n0, n1, hm = 128, 96, 2
-    rng = np.random.default_rng(11)
+@pytest.mark.parametrize("n0,n1,hm", [(128, 96, 2), (128, 128, 3)])
+def test_2d_in_place(lf, oracle, n0, n1, hm):
+    rng = np.random.default_rng(11 + n1)
     x = rng.standard_normal((hm, n0, n1)) + 1j * rng.standard_normal((hm, n0, n1))
     plan = lf.Plan2D(n0, n1, lf.FORWARD, hm)
     xd = torch.from_numpy(x).to(DEV)
