@@ -341,11 +341,26 @@ struct Split2dSchedule {
   const void *k[2];
 };
 
+using R84 = Radices<8, 4>;
+using R88 = Radices<8, 8>;
 using R882 = Radices<8, 8, 2>;
+using R884 = Radices<8, 8, 4>;
+// arrays of 8192 and 16384 points whose columns and rows each fit one wave (16 elements per
+// thread), with the default radices of their parts
+template <class RA, class RB>
+constexpr Split2dSchedule s2d(int n0, int n1, int na, const int (&ra)[3], int nb, const int (&rb)[3]) {
+  return {n0, n1, 16, na, nb, {ra[0], ra[1], ra[2], 0}, {rb[0], rb[1], rb[2], 0},
+          (size_t)split_lds_doubles(RA{}, RB{}) * sizeof(double),
+          {(const void *)split2d_kernel<false, 16, RA, RB>, (const void *)split2d_kernel<true, 16, RA, RB>}};
+}
 const Split2dSchedule kSplit2d[] = {
-    {128, 128, 16, 3, 3, {8, 8, 2}, {8, 8, 2}, (size_t)split_lds_doubles(R882{}, R882{}) * sizeof(double),
-     {(const void *)split2d_kernel<false, 16, R882, R882>,
-      (const void *)split2d_kernel<true, 16, R882, R882>}},
+    s2d<R882, R882>(128, 128, 3, {8, 8, 2}, 3, {8, 8, 2}),
+    s2d<R88, R884>(64, 256, 2, {8, 8, 0}, 3, {8, 8, 4}),
+    s2d<R884, R88>(256, 64, 3, {8, 8, 4}, 2, {8, 8, 0}),
+    s2d<R88, R882>(64, 128, 2, {8, 8, 0}, 3, {8, 8, 2}),
+    s2d<R882, R88>(128, 64, 3, {8, 8, 2}, 2, {8, 8, 0}),
+    s2d<R84, R884>(32, 256, 2, {8, 4, 0}, 3, {8, 8, 4}),
+    s2d<R884, R84>(256, 32, 3, {8, 8, 4}, 2, {8, 4, 0}),
 };
 
 struct SplitSchedule {

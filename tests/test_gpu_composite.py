@@ -140,9 +140,11 @@ def oracle_2d(lf, oracle, fwd, x):
     return oracle.ref_transform(fwd, rs1, rows).reshape(hm, n0, n1)
 
 
-# (128, 128, 5): the single-pass register kernel (both axes in one workgroup)
+# (128, 128), (64, 256), (256, 64), (64, 128), (128, 64), (32, 256), (256, 32): the
+# single-pass register kernel (both axes in one workgroup)
 SHAPES_2D = [(2, 2, 1), (3, 5, 2), (8, 8, 3), (16, 1000, 1), (64, 64, 4), (12, 4096, 1),
-             (128, 128, 5), (256, 256, 2), (1000, 12, 1), (1024, 1024, 1)]
+             (128, 128, 5), (64, 256, 3), (256, 64, 3), (64, 128, 4), (128, 64, 4), (32, 256, 3),
+             (256, 32, 3), (256, 256, 2), (1000, 12, 1), (1024, 1024, 1)]
 
 
 @pytest.mark.parametrize("n0,n1,hm", SHAPES_2D)
@@ -159,7 +161,7 @@ def test_2d_bit_exact(lf, oracle, n0, n1, hm, fwd):
     assert rel_l2(got, ref) < TOL
 
 
-@pytest.mark.parametrize("n0,n1,hm", [(128, 96, 2), (128, 128, 3)])
+@pytest.mark.parametrize("n0,n1,hm", [(128, 96, 2), (128, 128, 3), (256, 32, 2)])
 def test_2d_in_place(lf, oracle, n0, n1, hm):
     rng = np.random.default_rng(11 + n1)
     x = rng.standard_normal((hm, n0, n1)) + 1j * rng.standard_normal((hm, n0, n1))
