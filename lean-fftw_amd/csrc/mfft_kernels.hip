@@ -11,7 +11,9 @@
 
 namespace mfft {
 
-template <bool FWD, int RSET, int EPT, int MAXT, bool SQ>
+// PS: input weights scaled by a device scalar (the adjoint of the differentiable path), a
+// variant of its own so that the other instances keep their registers.
+template <bool FWD, int RSET, int EPT, int MAXT, bool SQ, bool PS>
 __global__ __launch_bounds__(MAXT) void stage_kernel(const double2 *in, double2 *out,
                                                      const double2 *__restrict__ tw,
                                                      const StageArgs a) {
@@ -23,10 +25,12 @@ __global__ __launch_bounds__(MAXT) void stage_kernel(const double2 *in, double2 
               a.in_e, a.pre ? a.pre + g0 * a.pre_g : nullptr, a.pre_g, a.pre_e, a.cached_in};
   HbmSide hout{out + z * a.out_z + h * a.out_h + g0 * a.out_g, nullptr, a.out_g, a.out_e,
                a.post ? a.post + g0 * a.post_g : nullptr, a.post_g, a.post_e, a.cached_out};
-  if (a.pre_s) hin.wk = a.pre_k * a.pre_s[0];
+  if constexpr (PS) {
+    if (a.pre_s) hin.wk = a.pre_k * a.pre_s[0];
+  }
   // (the run-time-sided pass body of run_stage_compact measured the same here: MI355X, mixed
   // sizes on this kernel 2.91 / 2.92 TB/s)
-  const double part = run_stage<FWD, RSET, EPT, SQ>(lds, hin, hout, threadIdx.x, blockDim.x, a.G, gcount, a.L,
+  const double part = run_stage<FWD, RSET, EPT, SQ, PS>(lds, hin, hout, threadIdx.x, blockDim.x, a.G, gcount, a.L,
                  a.npass, a.pass, a.gfast_in, a.gfast_out, tw + h * a.kb_h, g0, a.kb_u, a.stage_io, a.m_L);
   if constexpr (SQ) {
     // deterministic per-workgroup partial of sum |out|^2: a fixed reduction tree
@@ -44,10 +48,13 @@ __global__ __launch_bounds__(MAXT) void stage_kernel(const double2 *in, double2 
   }
 }
 
-// the kernel instance for (direction, radix set, elements per thread, sum of squares)
-const void *kernel_for(int fwd, int set, int ept, int sq, int big) {
+// the kernel instance for (direction, radix set, elements per thread, sum of squares, scaled
+// input weights; sum-of-squares calls take no scaled weights)
+const void *kernel_for(int fwd, int set, int ept, int sq, int big, int ps) {
 #define MFFT_K(F, S, E, T)                                                                   \
-  (sq ? (const void *)stage_kernel<F, S, E, T, true> : (const void *)stage_kernel<F, S, E, T, false>)
+  (sq ? (const void *)stage_kernel<F, S, E, T, true, false>                                 \
+      : ps ? (const void *)stage_kernel<F, S, E, T, false, true>                            \
+           : (const void *)stage_kernel<F, S, E, T, false, false>)
 #define MFFT_DIR(S, E, T) (fwd ? MFFT_K(true, S, E, T) : MFFT_K(false, S, E, T))
   if (ept == 16) {
     if (set == RSET_POW2) return MFFT_DIR(RSET_POW2, 16, 512);
@@ -93,15 +100,17 @@ hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const doub
                   : small               ? RSET_SMALL
                   : (mid && a.ept == 8) ? RSET_MID
                                         : RSET_ALL;
-  const void *k = kernel_for(fwd, set, a.ept, a.sumsq != nullptr, a.threads > 256);
+  const void *k = kernel_for(fwd, set, a.ept, a.sumsq != nullptr, a.threads > 256,
+                             a.pre && a.pre_s && !a.sumsq);
   static const bool attr = [] {
     for (int f = 0; f < 2; ++f)
       for (int st = 0; st < 4; ++st)
         for (int e = 8; e <= 16; e += 8)
           for (int q = 0; q < 2; ++q)
             for (int b = 0; b < 2; ++b)
-              (void)hipFuncSetAttribute(kernel_for(f, st, e, q, b),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytesMax);
+              for (int ps = 0; ps < 2; ++ps)
+                (void)hipFuncSetAttribute(kernel_for(f, st, e, q, b, ps),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytesMax);
     return true;
   }();
   (void)attr;

@@ -98,7 +98,9 @@ __device__ __forceinline__ bool stage_block(int xcd, int64_t nsub, int G, int &b
 // Where a pass reads or writes in HBM: element e of sub-problem g of the tile is
 // base[off(g) + e * stride], off(g) = list ? list[g] : g * gstride, optionally scaled by a
 // real weight wt[g * wg + e * wstride] (weights are shared across the batch); on the input
-// side the weight is wk * wt[...] (wk = 1 unless the input weights carry a scalar factor).
+// side of the kernels compiled with PS (a scaled input weight) the weight is wk * wt[...].
+// Only those kernels read wk: the others keep the registers it would cost (the generic stage
+// kernel of the mid radix set, 118 -> 134 VGPRs and 109 -> 128 plus spills, at the time).
 // With SQ the last pass also returns the sum of |x|^2 of the elements the thread stored.
 struct HbmSide {
   double2 *base;
@@ -117,7 +119,7 @@ __device__ __forceinline__ int64_t hbm_off(const HbmSide &s, int g) {
 // One pass; src_hbm / dst_hbm: the inputs come from HBM / the outputs go to HBM (else the LDS
 // buffer). pass_run fixes them at compile time; the ragged kernel passes them at run time, so
 // that it holds one copy of each radix's pass instead of four.
-template <bool FWD, int R, int EPT, bool SQ>
+template <bool FWD, int R, int EPT, bool SQ, bool PS = false>
 __device__ __forceinline__ double pass_body(double2 *buf, const HbmSide &hin, const HbmSide &hout,
                                           int tid, int nthreads, int G, int gcount, int L,
                                           const PassArgs &pa, bool gfast,
@@ -165,7 +167,7 @@ __device__ __forceinline__ double pass_body(double2 *buf, const HbmSide &hin, co
           const int64_t ws = (int64_t)LR * hin.wstride;
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            const double v = hin.wk * w[r * ws];
+            const double v = PS ? hin.wk * w[r * ws] : w[r * ws];
             x[u][r].x = x[u][r].x * v;
             x[u][r].y = x[u][r].y * v;
           }
@@ -226,19 +228,19 @@ __device__ __forceinline__ double pass_body(double2 *buf, const HbmSide &hin, co
   return sumsq;
 }
 
-template <bool FWD, int R, bool SRC_HBM, bool DST_HBM, int EPT, bool SQ>
+template <bool FWD, int R, bool SRC_HBM, bool DST_HBM, int EPT, bool SQ, bool PS = false>
 __device__ __forceinline__ double pass_run(double2 *buf, const HbmSide &hin, const HbmSide &hout,
                                          int tid, int nthreads, int G, int gcount, int L,
                                          const PassArgs &pa, bool gfast,
                                          const double2 *__restrict__ tw, int64_t kb0,
                                          int64_t kb_u) {
-  return pass_body<FWD, R, EPT, SQ>(buf, hin, hout, tid, nthreads, G, gcount, L, pa, gfast, tw,
+  return pass_body<FWD, R, EPT, SQ, PS>(buf, hin, hout, tid, nthreads, G, gcount, L, pa, gfast, tw,
                                     kb0, kb_u, SRC_HBM, DST_HBM);
 }
 
 // Coalesced copy of a tile of contiguous sub-problems between HBM and LDS: consecutive lanes
 // take consecutive elements f = g * L + i (sub-problem g at g * gstride, gstride == L).
-template <int EPT>
+template <int EPT, bool PS = false>
 __device__ __forceinline__ void tile_to_lds(double2 *buf, const HbmSide &h, int tid, int nthreads,
                                             int gcount, int L, uint32_t m_L) {
   const int total = gcount * L, ll = lds_len(L);
@@ -250,7 +252,8 @@ __device__ __forceinline__ void tile_to_lds(double2 *buf, const HbmSide &h, int 
       v[u] = ld_stream(h.base + f);
       if (h.wt) {
         const int g = (int)fdiv((uint32_t)f, m_L);
-        const double w = h.wk * h.wt[(int64_t)g * h.wg + (int64_t)(f - g * L) * h.wstride];
+        const double w = PS ? h.wk * h.wt[(int64_t)g * h.wg + (int64_t)(f - g * L) * h.wstride]
+                            : h.wt[(int64_t)g * h.wg + (int64_t)(f - g * L) * h.wstride];
         v[u].x = v[u].x * w;
         v[u].y = v[u].y * w;
       }
@@ -297,7 +300,7 @@ __device__ __forceinline__ double tile_from_lds(const double2 *buf, const HbmSid
 // to LDS and finally back to HBM by coalesced tile copies (for contiguous small
 // sub-problems, whose butterfly reads would otherwise leave lanes far apart). Twiddle index
 // of pass p for local index k of sub-problem g: (kb0 + g) * kb_u + k * kmul.
-template <bool FWD, int RSET, int EPT, bool SQ = false>
+template <bool FWD, int RSET, int EPT, bool SQ = false, bool PS = false>
 __device__ __forceinline__ double run_stage(double2 *buf, const HbmSide &hin, const HbmSide &hout,
                                           int tid, int nthreads, int G, int gcount, int L,
                                           int npass, const PassArgs *pas, int gfast_in,
@@ -305,7 +308,7 @@ __device__ __forceinline__ double run_stage(double2 *buf, const HbmSide &hin, co
                                           int64_t kb0, int64_t kb_u, int stage_io = 0,
                                           uint32_t m_L = 0) {
   double sumsq = 0.0;
-  if (stage_io) tile_to_lds<EPT>(buf, hin, tid, nthreads, gcount, L, m_L);
+  if (stage_io) tile_to_lds<EPT, PS>(buf, hin, tid, nthreads, gcount, L, m_L);
   for (int p = 0; p < npass; ++p) {
     const PassArgs pa = pas[p];
     const bool first = p == 0 && !stage_io, last = p == npass - 1 && !stage_io;
@@ -314,19 +317,19 @@ __device__ __forceinline__ double run_stage(double2 *buf, const HbmSide &hin, co
   case RR:                                                                                       \
     if (!in_set(RSET, RR)) break;                                                                \
     if (first && last)                                                                           \
-      sumsq = pass_run<FWD, RR, true, true, EPT, SQ>(buf, hin, hout, tid, nthreads, G, gcount, L, \
+      sumsq = pass_run<FWD, RR, true, true, EPT, SQ, PS>(buf, hin, hout, tid, nthreads, G, gcount, L, \
         pa, gfast, tw,  \
                                      kb0, kb_u);                                             \
     else if (first)                                                                              \
-      pass_run<FWD, RR, true, false, EPT, SQ>(buf, hin, hout, tid, nthreads, G, gcount, L, \
+      pass_run<FWD, RR, true, false, EPT, SQ, PS>(buf, hin, hout, tid, nthreads, G, gcount, L, \
         pa, gfast, tw,  \
                                      kb0, kb_u);                                     \
     else if (last)                                                                               \
-      sumsq = pass_run<FWD, RR, false, true, EPT, SQ>(buf, hin, hout, tid, nthreads, G, gcount, L, \
+      sumsq = pass_run<FWD, RR, false, true, EPT, SQ, PS>(buf, hin, hout, tid, nthreads, G, gcount, L, \
         pa, gfast, tw,  \
                                      kb0, kb_u);                                     \
     else                                                                                         \
-      pass_run<FWD, RR, false, false, EPT, SQ>(buf, hin, hout, tid, nthreads, G, gcount, L, \
+      pass_run<FWD, RR, false, false, EPT, SQ, PS>(buf, hin, hout, tid, nthreads, G, gcount, L, \
         pa, gfast, tw,  \
                                      kb0, kb_u);                                    \
     break;

@@ -126,8 +126,10 @@ constexpr int reg_block() {
   return T < 256 ? 256 : T;
 }
 
-// Load a transform's columns t + T m (non-temporal), times the input weights.
-template <int T, int E, bool REAL16 = false>
+// Load a transform's columns t + T m (non-temporal), times the input weights (scaled by the
+// device scalar pre_s only when PS: the fixed kernels without it keep 60-64 VGPRs at 8
+// elements per thread, which a scaled weight took to 68-72, and no spills at 16).
+template <int T, int E, bool REAL16 = false, bool PS = true>
 __device__ __forceinline__ void reg_load(double2 (&v)[E], const StageArgs &a, const double2 *in,
                                          int64_t base, int t, bool valid) {
   // 16 elements per thread: real input only, and only in the kernels that take it without
@@ -155,12 +157,21 @@ __device__ __forceinline__ void reg_load(double2 (&v)[E], const StageArgs &a, co
       v[m] = valid ? load_io(a, in, base, (int64_t)(t + T * m) * a.in_e) : make_double2(0.0, 0.0);
   }
   if (a.pre) {
-    const double wk = a.pre_s ? a.pre_k * a.pre_s[0] : 1.0;
+    if constexpr (PS) {
+      const double wk = a.pre_s ? a.pre_k * a.pre_s[0] : 1.0;
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const double w = wk * a.pre[(int64_t)(t + T * m) * a.pre_e];
-      v[m].x = v[m].x * w;
-      v[m].y = v[m].y * w;
+      for (int m = 0; m < E; ++m) {
+        const double w = wk * a.pre[(int64_t)(t + T * m) * a.pre_e];
+        v[m].x = v[m].x * w;
+        v[m].y = v[m].y * w;
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const double w = a.pre[(int64_t)(t + T * m) * a.pre_e];
+        v[m].x = v[m].x * w;
+        v[m].y = v[m].y * w;
+      }
     }
   }
 }
@@ -299,7 +310,7 @@ constexpr int first_radix() {
 // Below 2048 points a workgroup runs G = 256 / T transforms, each with its own LDS region,
 // and exchanges whole complex values (LDS is not the limit there); from 2048 on one transform
 // and the real/imaginary exchange.
-template <bool FWD, int T, int E, bool SQ, int... Rs>
+template <bool FWD, int T, int E, bool SQ, bool PS, int... Rs>
 __global__ __launch_bounds__(reg_block<T>(), 4) void fixed_kernel(const double2 *in, double2 *out,
                                                                  const double2 *__restrict__ tw,
                                                                  const StageArgs a) {
@@ -312,25 +323,28 @@ __global__ __launch_bounds__(reg_block<T>(), 4) void fixed_kernel(const double2 
   const int64_t h = (int64_t)blockIdx.x * G + g;
   const bool valid = G == 1 || h < a.nsub;  // one transform per workgroup: the grid is the batch
   double2 v[E];
-  reg_load<T, E, first_radix<Rs...>() == 16>(v, a, in, (valid ? h : 0) * a.in_g, t, valid);
+  reg_load<T, E, first_radix<Rs...>() == 16, PS>(v, a, in, (valid ? h : 0) * a.in_g, t, valid);
   fixed_passes<FWD, T, E, HALF, 0, 1, Rs...>(v, t, lds_d, a, tw);
   const double sumsq = reg_store<T, E>(v, a, out, (valid ? h : 0) * a.out_g, t, valid);
   if constexpr (SQ) block_partial<reg_block<T>()>(sumsq, a.sumsq + blockIdx.x);
 }
 
 // The fixed schedules: N, elements per thread, radix sequence, and the kernels by
-// (direction, sum of squares).
+// (direction, sum of squares, input weights scaled by a device scalar; sum-of-squares calls
+// take no scaled weights).
 struct FixedSchedule {
   int n, e, npass;
   int rs[8];
-  const void *k[2][2];
+  const void *k[2][2][2];
 };
 
 #define MFFT_FX(TT, EE, ...)                                                                 \
-  {{(const void *)fixed_kernel<false, TT, EE, false, __VA_ARGS__>,                            \
-    (const void *)fixed_kernel<false, TT, EE, true, __VA_ARGS__>},                            \
-   {(const void *)fixed_kernel<true, TT, EE, false, __VA_ARGS__>,                             \
-    (const void *)fixed_kernel<true, TT, EE, true, __VA_ARGS__>}}
+  {{{(const void *)fixed_kernel<false, TT, EE, false, false, __VA_ARGS__>,                    \
+     (const void *)fixed_kernel<false, TT, EE, false, true, __VA_ARGS__>},                    \
+    {(const void *)fixed_kernel<false, TT, EE, true, false, __VA_ARGS__>, nullptr}},          \
+   {{(const void *)fixed_kernel<true, TT, EE, false, false, __VA_ARGS__>,                     \
+     (const void *)fixed_kernel<true, TT, EE, false, true, __VA_ARGS__>},                     \
+    {(const void *)fixed_kernel<true, TT, EE, true, false, __VA_ARGS__>, nullptr}}}
 const FixedSchedule kFixed[] = {
     {64, 8, 2, {8, 8}, MFFT_FX(8, 8, 8, 8)},
     {128, 8, 3, {8, 8, 2}, MFFT_FX(16, 8, 8, 8, 2)},
@@ -618,7 +632,9 @@ hipError_t launch_reg(bool fwd, const StageArgs &a, const double2 *in, double2 *
   int rs[MAX_PASSES];
   for (int p = 0; p < a.npass; ++p) rs[p] = a.pass[p].R;
   if (const FixedSchedule *f = fixed_schedule(a.L, rs, a.npass)) {
-    const void *k = f->k[fwd ? 1 : 0][a.sumsq ? 1 : 0];
+    const bool ps = a.pre && a.pre_s;
+    if (ps && a.sumsq) return hipErrorInvalidValue;  // no such call reaches here
+    const void *k = f->k[fwd ? 1 : 0][a.sumsq ? 1 : 0][ps ? 1 : 0];
     const int T = a.L / f->e;
     const int BT = T < 256 ? 256 : T, G = BT / T;  // as reg_block<T>()
     const size_t lds = G == 1 ? (size_t)reg_lds_doubles(a.L) * sizeof(double)
