@@ -75,6 +75,9 @@ namespace {
 // planner flag of the complex parts of real-data plans: register-kernel schedules into whose
 // loads and stores the real-data maps fuse (not part of the public flags)
 constexpr unsigned kFuseRadices = 1u << 30;
+// The largest power of two whose real-data plan stages its tile with the maps in the tile
+// copies; the register kernels take the maps into their own loads and stores above it.
+constexpr int64_t kFuseTileMax = 64;
 
 std::atomic<size_t> g_threads{1};
 std::mutex g_wisdom_mu;
@@ -549,11 +552,12 @@ bool build_stages(mfft_plan *p, int split, int split2 = 0) {
   std::vector<int64_t> ns(s + 1, 1);
   for (int t = 0; t < s; ++t) ns[t + 1] = ns[t] * rs[t];
   if (split <= 0 || split >= s) {
-    // the complex part of a real-data plan runs a staged tile kernel up to 64 points, whose
-    // tile copies take the real-data maps (the register kernel's fused maps from 128 on)
-    constexpr int64_t fuse_tile_max = 64;
+    // the complex part of a real-data plan runs a staged tile kernel, whose tile copies take
+    // the real-data maps: powers of two up to 64 points (the register kernel's fused maps from
+    // 128 on), every other single-stage size up to a tile of LDS
+    const bool pow2 = (N & (N - 1)) == 0;
     const bool reg = p->allow_big && reg_size(N) && p->istride == 1 && p->ostride == 1 &&
-                     mfft::reg_supported(N, rs.data(), s) && !(p->fuse && N <= fuse_tile_max);
+                     mfft::reg_supported(N, rs.data(), s) && !(p->fuse && N <= kFuseTileMax);
     const bool big = !reg && N > mfft::LDS_ELEMS && p->allow_big &&
                      N <= ((N & (N - 1)) == 0 ? mfft::BIG_ELEMS : mfft::BIG_ELEMS / 2);
     if ((N > mfft::LDS_ELEMS && !big && !reg) || s > mfft::MAX_PASSES) return false;
@@ -590,13 +594,18 @@ bool build_stages(mfft_plan *p, int split, int split2 = 0) {
     a.pre_e = 1;
     a.post_g = 0;
     a.post_e = 1;
-    // small contiguous transforms: stage the tile through LDS with coalesced copies
+    // small contiguous transforms: stage the tile through LDS with coalesced copies; the
+    // complex parts of real-data plans whose sizes are not powers of two, up to the largest
+    // tile kernel (4096 points), with the maps in the copies (a stage that finds no tile
+    // kernel drops them again, untiled())
     a.m_L = mfft::magic32(N);
-    const int64_t io_max = p->fuse ? std::max<int64_t>(fuse_tile_max, stage_io_max()) : stage_io_max();
-    a.stage_io = (N <= io_max && a.ept == 8 && p->istride == 1 && p->ostride == 1 &&
-                  p->idist == N && p->odist == N)
-                     ? (p->fuse ? 2 : 1)
-                     : 0;
+    const bool contiguous = p->istride == 1 && p->ostride == 1 && p->idist == N && p->odist == N;
+    const int64_t io_max = p->fuse && pow2 ? std::max<int64_t>(kFuseTileMax, stage_io_max())
+                                           : stage_io_max();
+    if (contiguous && p->fuse && !pow2 && N <= 2 * (int64_t)mfft::LDS_ELEMS)
+      a.stage_io = 2;
+    else
+      a.stage_io = contiguous && N <= io_max && a.ept == 8 ? (p->fuse ? 2 : 1) : 0;
     // the staged tile is read with ordinary loads (MI355X, tile kernels, 256 MB batches of
     // 2..16 points: 87-90 us with non-temporal loads, 80-82 with ordinary ones)
     if (a.stage_io) a.cached_in = 1;
@@ -686,6 +695,14 @@ bool build_stages(mfft_plan *p, int split, int split2 = 0) {
   return true;
 }
 
+// A stage left without a tile kernel (a batch smaller than the tile, a measured schedule
+// without one): above kFuseTileMax points the stage kernel reads it butterfly by butterfly, as
+// it reads every complex stage of that size, and the real-data plan runs the element maps.
+void untiled(StageArgs &st) {
+  st.tile = 0;
+  if (st.stage_io == 2 && st.L > kFuseTileMax) st.stage_io = 0;
+}
+
 mfft_plan *finish(mfft_plan *p, int split, int split2 = 0) {
   if (!build_stages(p, split, split2) || hipGetDevice(&p->device) != hipSuccess ||
       build_twiddles(p) != hipSuccess) {
@@ -694,7 +711,10 @@ mfft_plan *finish(mfft_plan *p, int split, int split2 = 0) {
   }
   // stages with a fixed-schedule tile kernel run it (two-stage column plans size their stages
   // per chunk and look theirs up at run time)
-  for (auto &st : p->stages) st.tile = p->columns ? 0 : mfft::tile_lookup(st);
+  for (auto &st : p->stages) {
+    st.tile = p->columns ? 0 : mfft::tile_lookup(st);
+    if (!st.tile) untiled(st);
+  }
   if (p->stages.size() == 3) p->stages[1].tile = 0;  // the middle stage needs grid.z
   if (p->stages.size() > 1) {
     const size_t per = (p->stages.size() == 3 ? 2 : 1) * (size_t)p->n;  // intermediates
@@ -1221,7 +1241,7 @@ void apply_tiles(mfft_plan *q, const Cand &c) {
     StageArgs &st = q->stages[s];
     if (c.tg[s] < 0) continue;
     if (c.tg[s] == 0) {
-      st.tile = 0;
+      untiled(st);
       continue;
     }
     int tiles[16];

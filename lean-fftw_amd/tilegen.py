@@ -6,9 +6,10 @@ that is not a power of two (csrc/mfft_tile_gen0.hip ... mfft_tile_gen7.hip).
 Each entry is the single-stage contiguous shape of the size's default plan: the radices come
 from the library's own planner (mfft_plan_schedule, a host-only query), the tile holds
 G = 2048 // n transforms (one above 2048 points) on 256 threads, sizes up to 16 points stage
-the tile through LDS (as the planner does), and sizes up to 64 points also get the staged
-variant that takes the real-data maps (the complex parts of r2c / c2r plans). Shapes already
-in the hand-tuned table of mfft_tile.hip are skipped. tests/test_abi.py checks that the
+the tile through LDS (as the planner does), and every size also gets the staged variant that
+takes the real-data maps (the complex parts of r2c / c2r plans: the real input or Hermitian
+half read, the half or real output written, in the tile copies). Shapes already in the
+hand-tuned table of mfft_tile.hip are skipped. tests/test_abi.py checks that the
 committed tables still match the planner.
 """
 
@@ -22,7 +23,6 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 PARTS = 8
 STAGE_IO_MAX = 16   # the planner's staged-tile limit for plain complex plans
-FUSE_TILE_MAX = 64  # ... and for the complex parts of real-data plans
 # shapes of the hand table (mfft_tile.hip): (n, io)
 HAND = {(6, 1), (6, 2), (60, 0), (360, 0), (1000, 0), (2100, 0)}
 # sizes whose tile measured slower than the stage kernel on MI355X (64 MB batches: 20, 25 and 36
@@ -56,9 +56,7 @@ def entries(lib):
         if split:
             continue
         G = max(1, 2048 // n)
-        ios = [1 if n <= STAGE_IO_MAX else 0]
-        if n <= FUSE_TILE_MAX:
-            ios.append(2)
+        ios = [1 if n <= STAGE_IO_MAX else 0, 2]
         for io in ios:
             if (n, io) not in HAND and (n, io) not in SLOWER:
                 out.append((n, rs, io, G, 256))
