@@ -517,7 +517,9 @@ def bench_real(args, torch, lf):
             "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": round(tot_t * 1e3, 4), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-            "config": {"workload": "r2c FP64, N = 2^4..2^18, batch 2^24/N, out of place"},
+            "config": {"workload": ("r2c FP64, N = 2^4..2^18" if not args.sizes else
+                                    "r2c FP64, N in " + ", ".join(str(n) for n in sizes))
+                                   + ", batch 2^24/N, out of place"},
             "per_size": per}
 
 

@@ -70,8 +70,8 @@ def test_static_matches_execute_two_calls(lf):
 
 def test_static_single_stage_and_composites(lf):
     """Single-stage plans (no intermediate), a two-stage column layout, a 2-D plan whose
-    columns run two stages, and a real-data plan whose maps fuse; an unfused real-data plan
-    is refused with MFFT_ERR_UNSUPPORTED."""
+    columns run two stages, and real-data plans whose maps fuse (into register kernels, into
+    a staged tile); an unfused real-data plan is refused with MFFT_ERR_UNSUPPORTED."""
     x = rand((1000, 1024), 5)
     check(lf, lf.Plan(1024, lf.FORWARD, 1000), x, x)
     # columns of a 8192 x 64 row-major array: 64 transforms of 8192 points, stride 64
@@ -85,9 +85,12 @@ def test_static_single_stage_and_composites(lf):
     pr = lf.RealPlan(65536, lf.FORWARD, 40)
     out = torch.empty((40, 65536 // 2 + 1), dtype=torch.complex128, device=DEV)
     check(lf, pr, r, out)
-    pu = lf.RealPlan(360, lf.FORWARD, 8)  # no fused maps for 360 points
-    ru = rand((8, 360), 9, torch.float64)
-    outu = torch.empty((8, 181), dtype=torch.complex128, device=DEV)
+    pt = lf.RealPlan(360, lf.FORWARD, 8)  # the maps in a staged tile's copies
+    rt = rand((8, 360), 9, torch.float64)
+    check(lf, pt, rt, torch.empty((8, 181), dtype=torch.complex128, device=DEV))
+    pu = lf.RealPlan(10368, lf.FORWARD, 2)  # two stages in tile kernels: no fused maps
+    ru = rand((2, 10368), 11, torch.float64)
+    outu = torch.empty((2, 5185), dtype=torch.complex128, device=DEV)
     assert abi_call(lf, "mfft_execute_static", pu, ru, outu) == 2  # MFFT_ERR_UNSUPPORTED
 
 
