@@ -52,7 +52,7 @@ __device__ __forceinline__ void st_stream(double2 *p, double2 v) {
 // elements of the array's own type). Input: io_in 1 reads doubles as (x, 0); 2 reads the
 // Hermitian extension of n/2 + 1 complex values (conj of element n - off above n/2; cached
 // loads, since each half element is read twice). Output: io_out 1 stores real parts as
-// doubles; 2 stores the first n/2 + 1 elements only.
+// doubles; 2 stores the first n/2 + 1 elements only; non-temporal stores unless cached_out.
 __device__ __forceinline__ double2 load_io(const StageArgs &a, const double2 *in, int64_t base,
                                            int64_t off) {
   if (a.io_in == 1)
@@ -66,10 +66,18 @@ __device__ __forceinline__ double2 load_io(const StageArgs &a, const double2 *in
 
 __device__ __forceinline__ void store_io(const StageArgs &a, double2 *out, int64_t base,
                                          int64_t off, double2 y) {
-  if (a.io_out == 1)
-    __builtin_nontemporal_store(y.x, reinterpret_cast<double *>(out) + base + off);
-  else if (off <= a.io_n / 2)
-    st_stream(out + base + off, y);
+  if (a.io_out == 1) {
+    double *d = reinterpret_cast<double *>(out) + base + off;
+    if (a.cached_out)
+      *d = y.x;
+    else
+      __builtin_nontemporal_store(y.x, d);
+  } else if (off <= a.io_n / 2) {
+    if (a.cached_out)
+      out[base + off] = y;
+    else
+      st_stream(out + base + off, y);
+  }
 }
 
 // q = n / d for n * d < 2^32, with m = ceil(2^32 / d) (m = 0 encodes d = 1)
