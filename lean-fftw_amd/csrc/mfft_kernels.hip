@@ -75,8 +75,10 @@ hipError_t launch_stage(bool fwd, const StageArgs &a, int64_t ntrans, const doub
   if (nz > 1 && (a.tile > 0 || a.reg != 0 || a.sumsq || nz > 65535 || ntrans > 65535))
     return hipErrorInvalidValue;
   // the real-data element maps exist in the register kernels (input maps at 16 elements per
-  // thread: real input into a schedule opening with radix 16) and in the staged tile kernels
-  const bool tile_io = a.tile > 0 && a.stage_io == 2 && !a.pre && !a.post && !a.sumsq;
+  // thread: real input into a schedule opening with radix 16), in the staged tile kernels and
+  // in the tile kernels of two-stage real-data plans
+  const bool tile_io =
+      a.tile > 0 && (a.stage_io == 2 || a.stage_io == 3) && !a.pre && !a.post && !a.sumsq;
   if ((a.io_in || a.io_out) && !tile_io &&
       (a.reg == 0 || (a.reg == 2 && a.sumsq) ||
        (a.io_in && a.ept != 8 && !(a.io_in == 1 && a.reg == 1 && a.pass[0].R == 16))))

@@ -682,6 +682,11 @@ bool build_stages(mfft_plan *p, int split, int split2 = 0) {
   // ordinary loads (10368 points 257 -> 225 us, 165375 points 410 -> 357 per 256 MB batch)
   A.cached_out = 1;
   B.cached_in = B.reg == 0 ? 1 : 0;
+  // the complex part of a real-data plan in tile kernels: both stages take the tile variants
+  // with the maps in their HBM accesses (finish() drops them where a stage has none)
+  if (p->fuse && A.reg == 0 && B.reg == 0 && p->istride == 1 && p->ostride == 1 &&
+      p->idist == N && p->odist == N)
+    A.stage_io = B.stage_io = 3;
   p->stages.push_back(A);
   p->stages.push_back(B);
   p->stage_passes.push_back(split);
@@ -700,7 +705,7 @@ bool build_stages(mfft_plan *p, int split, int split2 = 0) {
 // it reads every complex stage of that size, and the real-data plan runs the element maps.
 void untiled(StageArgs &st) {
   st.tile = 0;
-  if (st.stage_io == 2 && st.L > kFuseTileMax) st.stage_io = 0;
+  if ((st.stage_io == 2 && st.L > kFuseTileMax) || st.stage_io == 3) st.stage_io = 0;
 }
 
 mfft_plan *finish(mfft_plan *p, int split, int split2 = 0) {
@@ -713,8 +718,20 @@ mfft_plan *finish(mfft_plan *p, int split, int split2 = 0) {
   // per chunk and look theirs up at run time)
   for (auto &st : p->stages) {
     st.tile = p->columns ? 0 : mfft::tile_lookup(st);
+    if (!st.tile && st.stage_io == 3) {  // no map variant: the stage's plain tile kernel
+      st.stage_io = 0;
+      st.tile = mfft::tile_lookup(st);
+    }
     if (!st.tile) untiled(st);
   }
+  // the maps of a two-stage plan fuse only when both stages carry them
+  if (p->stages.size() == 2 && (p->stages[0].stage_io == 3) != (p->stages[1].stage_io == 3))
+    for (auto &st : p->stages)
+      if (st.stage_io == 3) {
+        st.stage_io = 0;
+        st.tile = mfft::tile_lookup(st);
+        if (!st.tile) untiled(st);
+      }
   if (p->stages.size() == 3) p->stages[1].tile = 0;  // the middle stage needs grid.z
   if (p->stages.size() > 1) {
     const size_t per = (p->stages.size() == 3 ? 2 : 1) * (size_t)p->n;  // intermediates
@@ -851,13 +868,18 @@ int run(const mfft_plan *p, const double2 *in, double2 *out, const double *pre,
   return MFFT_SUCCESS;
 }
 
-// Whether every stage of a contiguous plan runs in a register kernel, which can take the
-// real-data element maps into its first loads and last stores (the input maps at 8 elements
-// per thread, or real input into a 16-per-thread schedule that opens with radix 16).
+// Whether the real-data element maps fuse into a contiguous plan's kernels: tile kernels that
+// carry them (a staged single stage, or both stages of a two-stage plan), or register kernels
+// in every stage, which take them into their first loads and last stores (the input maps at
+// 8 elements per thread, or real input into a 16-per-thread schedule that opens with radix 16).
 bool fusable(const mfft_plan *q, int io_in) {
   if (q->istride != 1 || q->ostride != 1 || q->columns || q->stages.empty()) return false;
-  // a single stage in a staged tile kernel with maps takes them into its tile copies
+  // a single stage in a staged tile kernel with maps takes them into its tile copies, two
+  // stages in tile kernels with maps into their first loads and last stores
   if (q->stages.size() == 1 && q->stages[0].tile > 0 && q->stages[0].stage_io == 2) return true;
+  if (q->stages.size() == 2 && q->stages[0].tile > 0 && q->stages[1].tile > 0 &&
+      q->stages[0].stage_io == 3 && q->stages[1].stage_io == 3)
+    return true;
   for (const auto &a : q->stages)
     if (a.reg == 0) return false;
   const StageArgs &a0 = q->stages[0];
@@ -902,7 +924,7 @@ int run_io(const mfft_plan *q, const void *in, void *out, int io_in, int io_out,
     // line) written with ordinary stores (MI355X, 2^24-point batches: 2^15 / 2^16 / 2^17 /
     // 2^18 points 168 / 169 / 192 / 218 -> 161 / 163 / 176 / 208 us); c2r's real rows keep
     // non-temporal stores (ordinary ones measured equal or up to 2% slower)
-    if (io_out == 2) B.cached_out = 1;
+    if (io_out == 2 && B.reg != 0) B.cached_out = 1;  // (the tile stages measured equal)
     const auto *src = (const double2 *)((const char *)in + (size_t)h0 * din * bin);
     auto *dst = (double2 *)((char *)out + (size_t)h0 * dout * bout);
     if (mfft::launch_stage(fwd, A, hc, src, w.ptr, q->d_tw, st) != hipSuccess ||

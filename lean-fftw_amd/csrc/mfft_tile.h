@@ -17,7 +17,9 @@
 // GFI / GFO: consecutive lanes walk the sub-problem index on the input / output side (the
 // sub-problems are adjacent in memory), else the butterfly index. IO: 0 the tile is read and
 // written by the first and last passes, 1 staged through LDS by coalesced tile copies, 2 staged
-// with the real-data maps of r2c / c2r plans in the tile copies.
+// with the real-data maps of r2c / c2r plans in the tile copies, 3 read and written by the
+// first and last passes with the real-data maps in those accesses (the stages of two-stage
+// real-data plans: stage A takes the input map, stage B the output map).
 #pragma once
 
 #include "mfft_pass.h"
@@ -34,11 +36,22 @@ __device__ __forceinline__ int lds_pos(int g, int e) {
     return e * CST + lpad(g);
 }
 
+// The real-data maps of an unstaged tile (IO 3): the raw arrays, the transform's start in
+// elements of each array's own type, and the tile's first sub-problem offset on each side, so
+// that an access's index within its transform is g0 offset + its offset from the tile's base.
+struct TileMaps {
+  const StageArgs *a;
+  const double2 *in;
+  double2 *out;
+  int64_t bin, bout, gin0, gout0;
+};
+
 template <bool FWD, int R, int L, int G, int NT, int NSL, bool SRC_HBM, bool DST_HBM, bool GF,
-          int CST = 0>
+          int CST = 0, bool MAPI = false, bool MAPO = false>
 __device__ __forceinline__ void tile_pass(double2 *buf, const HbmSide &hin, const HbmSide &hout,
                                           int tid, int gcount, const double2 *__restrict__ tw,
-                                          const PassArgs &pa, int64_t kb0, int64_t kb_u) {
+                                          const PassArgs &pa, int64_t kb0, int64_t kb_u,
+                                          const TileMaps *maps = nullptr) {
   constexpr int LR = L / R, NBT = G * LR, NB = (NBT + NT - 1) / NT, LL = lds_len(L);
   static_assert(L % R == 0, "radix must divide the sub-problem length");
   const bool twd = pa.twoff >= 0;
@@ -61,6 +74,15 @@ __device__ __forceinline__ void tile_pass(double2 *buf, const HbmSide &hin, cons
     js[u] = j;
     ok[u] = (NBT % NT == 0 || b < NBT) && g < gcount;
     if (ok[u]) {
+      if constexpr (SRC_HBM && MAPI) {
+        if (maps->a->io_in) {
+          const int64_t o = maps->gin0 + (int64_t)g * hin.gstride + (int64_t)j * hin.stride;
+          const int64_t st = (int64_t)LR * hin.stride;
+#pragma unroll
+          for (int r = 0; r < R; ++r) x[u][r] = load_io(*maps->a, maps->in, maps->bin, o + r * st);
+          continue;
+        }
+      }
       if constexpr (SRC_HBM) {
         const double2 *s = hin.base + (int64_t)g * hin.gstride + (int64_t)j * hin.stride;
         const int64_t st = (int64_t)LR * hin.stride;
@@ -91,6 +113,15 @@ __device__ __forceinline__ void tile_pass(double2 *buf, const HbmSide &hin, cons
     }
     Bfly<FWD, R, false>::run(x[u]);
     const int e0 = bb * NSL * R + k;
+    if constexpr (DST_HBM && MAPO) {
+      if (maps->a->io_out) {
+        const int64_t o = maps->gout0 + (int64_t)g * hout.gstride + (int64_t)e0 * hout.stride;
+        const int64_t st = (int64_t)NSL * hout.stride;
+#pragma unroll
+        for (int r = 0; r < R; ++r) store_io(*maps->a, maps->out, maps->bout, o + r * st, x[u][r]);
+        continue;
+      }
+    }
     if constexpr (DST_HBM) {
       double2 *d = hout.base + (int64_t)g * hout.gstride + (int64_t)e0 * hout.stride;
       const int64_t st = (int64_t)NSL * hout.stride;
@@ -114,14 +145,16 @@ template <bool FWD, int L, int G, int NT, int GFI, int GFO, int IO, int P, int N
           int... Rest>
 __device__ __forceinline__ void tile_passes(double2 *buf, const HbmSide &hin, const HbmSide &hout,
                                             int tid, int gcount, const StageArgs &a,
-                                            const double2 *__restrict__ tw, int64_t kb0) {
-  constexpr bool FIRST = P == 0 && !IO, LAST = sizeof...(Rest) == 0 && !IO;
+                                            const double2 *__restrict__ tw, int64_t kb0,
+                                            const TileMaps *maps) {
+  constexpr bool STAGED = IO == 1 || IO == 2;
+  constexpr bool FIRST = P == 0 && !STAGED, LAST = sizeof...(Rest) == 0 && !STAGED;
   constexpr bool GF = FIRST ? GFI != 0 : (LAST ? GFO != 0 : false);
-  tile_pass<FWD, R, L, G, NT, NSL, FIRST, LAST, GF>(buf, hin, hout, tid, gcount, tw, a.pass[P],
-                                                    kb0, a.kb_u);
+  tile_pass<FWD, R, L, G, NT, NSL, FIRST, LAST, GF, 0, FIRST && IO == 3, LAST && IO == 3>(
+      buf, hin, hout, tid, gcount, tw, a.pass[P], kb0, a.kb_u, maps);
   if constexpr (sizeof...(Rest) > 0)
     tile_passes<FWD, L, G, NT, GFI, GFO, IO, P + 1, NSL * R, Rest...>(buf, hin, hout, tid, gcount,
-                                                                       a, tw, kb0);
+                                                                       a, tw, kb0, maps);
 }
 
 // Coalesced copies of a tile of contiguous sub-problems (sub-problem g at g * L) between HBM
@@ -216,15 +249,18 @@ __global__ __launch_bounds__(NT) void tile_kernel(const double2 *in, double2 *ou
                      a.cached_out};
   const int tid = threadIdx.x;
   constexpr bool MAPS = IO == 2;
-  if constexpr (IO != 0) {
+  // IO 3: the maps in the first / last passes' accesses, which address the arrays in their
+  // own element types from the transform's start
+  TileMaps tm{&a, in, out, h * a.in_h, h * a.out_h, (int64_t)g0 * a.in_g, (int64_t)g0 * a.out_g};
+  if constexpr (IO == 1 || IO == 2) {
     // the real-data maps address the arrays in their own element types
     const size_t isz = MAPS && a.io_in == 1 ? sizeof(double) : sizeof(double2);
     const void *ib = reinterpret_cast<const char *>(in) + (h * a.in_h + (int64_t)g0 * a.in_g) * isz;
     tile_in<L, G, NT, MAPS>(lds, hin, tid, gcount, a, ib);
   }
   tile_passes<FWD, L, G, NT, GFI, GFO, IO, 0, 1, Rs...>(lds, hin, hout, tid, gcount, a,
-                                                        tw + h * a.kb_h, g0);
-  if constexpr (IO != 0) {
+                                                        tw + h * a.kb_h, g0, &tm);
+  if constexpr (IO == 1 || IO == 2) {
     const size_t osz = MAPS && a.io_out == 1 ? sizeof(double) : sizeof(double2);
     void *ob = reinterpret_cast<char *>(out) + (h * a.out_h + (int64_t)g0 * a.out_g) * osz;
     tile_out<L, G, NT, MAPS>(lds, hout, tid, gcount, a, ob);

@@ -74,6 +74,15 @@ const TileSchedule kTiles[] = {
     MFFT_T(280, 3, RS_(10, 7, 4), 1, 1, 0, 8, 256, 10, 7, 4),
     MFFT_T(375, 4, RS_(5, 5, 5, 3), 1, 0, 0, 8, 512, 5, 5, 5, 3),
     MFFT_T(441, 3, RS_(9, 7, 7), 1, 1, 0, 8, 512, 9, 7, 7),
+    // the same stages of the real-data plans of those sizes, with the maps in their first /
+    // last passes' accesses (stage A reads the real input or Hermitian half, stage B writes the
+    // half or the real parts)
+    MFFT_T(96, 2, RS_(12, 8), 1, 0, 3, 42, 512, 12, 8),
+    MFFT_T(108, 2, RS_(12, 9), 1, 1, 3, 36, 512, 12, 9),
+    MFFT_T(270, 3, RS_(10, 9, 3), 1, 0, 3, 8, 256, 10, 9, 3),
+    MFFT_T(280, 3, RS_(10, 7, 4), 1, 1, 3, 12, 512, 10, 7, 4),
+    MFFT_T(375, 4, RS_(5, 5, 5, 3), 1, 0, 3, 8, 512, 5, 5, 5, 3),
+    MFFT_T(441, 3, RS_(9, 7, 7), 1, 1, 3, 16, 1024, 9, 7, 7),
     // batches of adjacent transforms (the columns of a row-major array: lanes walk the
     // transforms on both sides), and the stages of two-stage column plans (2-D 64 x 64:
     // 173 -> 167 us, 4096 x 4096: 276 -> 267 per 2^24 elements; a tile for the 2048-point

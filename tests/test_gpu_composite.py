@@ -55,10 +55,12 @@ def herm_ext(h, n):
 # powers of two from 128 on run with the maps fused into the register kernels (16384 through
 # the embedded batch on the input side), batches that fill a staged tile (powers of two up to
 # 64 points, every other single-stage size up to 4096) with the maps in the tile copies
-# (60, 105, 1000, 2100 here; every generated shape in tests/test_gpu_tile.py); the others
-# (smaller batches, two-stage sizes that are not powers of two) through the element-map kernels
+# (60, 105, 1000, 2100 here; every generated shape in tests/test_gpu_tile.py), the two-stage
+# sizes 10368, 75600 and 165375 with the maps in their tile stages' first loads and last
+# stores; the others (smaller batches, other two-stage sizes that are not powers of two)
+# through the element-map kernels
 REAL_SIZES = [2, 3, 4, 5, 8, 15, 16, 32, 60, 64, 105, 128, 256, 512, 1000, 1024, 2048, 2100,
-              4096, 8192, 10368, 16384, 32768, 65536, 131072, 165375, 262144]
+              4096, 8192, 10368, 16384, 32768, 65536, 75600, 131072, 165375, 262144]
 
 
 @pytest.mark.parametrize("n", REAL_SIZES)
