@@ -88,9 +88,12 @@ def test_static_single_stage_and_composites(lf):
     pt = lf.RealPlan(360, lf.FORWARD, 8)  # the maps in a staged tile's copies
     rt = rand((8, 360), 9, torch.float64)
     check(lf, pt, rt, torch.empty((8, 181), dtype=torch.complex128, device=DEV))
-    pu = lf.RealPlan(10368, lf.FORWARD, 2)  # two stages in tile kernels: no fused maps
-    ru = rand((2, 10368), 11, torch.float64)
-    outu = torch.empty((2, 5185), dtype=torch.complex128, device=DEV)
+    p2 = lf.RealPlan(10368, lf.FORWARD, 3)  # the maps in two tile stages (an intermediate)
+    r2 = rand((3, 10368), 12, torch.float64)
+    check(lf, p2, r2, torch.empty((3, 5185), dtype=torch.complex128, device=DEV))
+    pu = lf.RealPlan(5000, lf.FORWARD, 2)  # two stages without map variants: no fused maps
+    ru = rand((2, 5000), 11, torch.float64)
+    outu = torch.empty((2, 2501), dtype=torch.complex128, device=DEV)
     assert abi_call(lf, "mfft_execute_static", pu, ru, outu) == 2  # MFFT_ERR_UNSUPPORTED
 
 
