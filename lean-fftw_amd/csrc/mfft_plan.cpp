@@ -959,6 +959,11 @@ int run_composite(const mfft_plan *p, const void *in, void *out, hipStream_t st,
     case 2: {  // c2r: Hermitian extension, backward DFT in place, real parts
       if (mfft::launch_herm_ext((const double2 *)in, p->d_work, p->n, hm, st) != hipSuccess)
         return MFFT_ERR_HIP;
+      // a single register kernel (16 elements per thread, where the Hermitian input map would
+      // spill) still takes the real-part map into its stores
+      const mfft_plan *q = p->part[0];
+      if (q->stages.size() == 1 && q->stages[0].reg == 1)
+        return run_io(q, p->d_work, out, 0, 1, st, nullptr);
       const int rc = run(p->part[0], p->d_work, p->d_work, nullptr, nullptr, nullptr, st);
       if (rc != MFFT_SUCCESS) return rc;
       return mfft::launch_real_part(p->d_work, (double *)out, p->n, hm, st) == hipSuccess
