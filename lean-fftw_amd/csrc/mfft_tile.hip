@@ -23,7 +23,8 @@ const TileSchedule kTiles[] = {
     MFFT_T(8, 1, RS_(8), 0, 0, 1, 256, 256, 8),
     MFFT_T(16, 1, RS_(16), 0, 0, 1, 128, 256, 16),
     MFFT_T(6, 1, RS_(6), 0, 0, 1, 256, 256, 6),
-    // the complex parts of real-data plans up to 64 points: staged tiles whose copies take
+    // the complex parts of real-data plans of powers of two up to 64 points and of 6 points
+    // (the generated tables hold every other size): staged tiles whose copies take
     // the real-data maps (tile copies 2; r2c per 256 MB of complex-equivalent batch on MI355X:
     // 16 and 32 points 221 / 207 -> 44 / 52 us against the unfused composition, 64 points
     // 65 -> 52 against the register kernel's fused maps, which are faster from 128 points)
