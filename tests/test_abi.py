@@ -135,6 +135,36 @@ def test_generated_tile_tables_are_current(lf):
         assert open(path).read() == tg.render(i, pe), path
 
 
+def test_real_data_stage_tiles_match_the_planner(lf):
+    """The hand table's tile variants with the real-data maps in their first / last passes
+    (csrc/mfft_tile.hip, tile copies 3) are the two stages of the default schedules of 10368,
+    75600 and 165375 points, with the lane orders of a two-stage plan (stage A reads columns,
+    stage B reads and writes rows), each beside its plain twin of the same tile; so the
+    real-data plans of those sizes fuse their maps rather than fall back to the composition."""
+    import re
+
+    src = open(os.path.join(ROOT, "lean-fftw_amd", "csrc", "mfft_tile.hip")).read()
+    pat = r"MFFT_T\((\d+), \d+, RS_\(([\d, ]+)\), (\d), (\d), (\d), (\d+), (\d+),"
+    table = [(int(m[0]), tuple(int(x) for x in m[1].split(",")), int(m[2]), int(m[3]), int(m[4]),
+              int(m[5]), int(m[6])) for m in re.findall(pat, src)]
+    mapped = {(L, rs, fi, fo, G, NT) for L, rs, fi, fo, io, G, NT in table if io == 3}
+    plain = {(L, rs, fi, fo, G, NT) for L, rs, fi, fo, io, G, NT in table if io == 0}
+    want = set()
+    for n in (10368, 75600, 165375):
+        rs, split = lf.schedule(n)
+        assert split, n
+        a, b = tuple(rs[:split]), tuple(rs[split:])
+        P = 1
+        for r in a:
+            P *= r
+        for stage, (L, rr, fi, fo) in (("A", (P, a, 1, 0)), ("B", (n // P, b, 1, 1))):
+            hit = [t for t in mapped if t[:4] == (L, rr, fi, fo)]
+            assert len(hit) == 1, (n, stage, L, rr)
+            want.add(hit[0])
+    assert mapped == want
+    assert mapped <= plain
+
+
 def test_missing_library_fails_loudly(lf, tmp_path, monkeypatch):
     monkeypatch.setattr(lf, "LIB_PATH", str(tmp_path / "nope.so"))
     monkeypatch.setattr(lf, "_lib", None)
