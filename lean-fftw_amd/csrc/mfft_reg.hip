@@ -599,10 +599,18 @@ hipError_t launch_reg_stage(bool fwd, const StageArgs &a, int64_t ntrans, const 
     if (done.insert(k).second)
       (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytesMax);
   }
-  // no XCD remap here: measured neutral to 4% slower for 2^15..2^18 points on MI355X
+  // XCD remap (neighbouring tiles of a row on one XCD) only for the real-data maps, whose rows
+  // of 8 doubles (real input) or of n/2 + 1 values per transform (the half output) leave
+  // 128-byte lines shared between neighbouring tiles, which then meet in one L2 (MI355X, r2c
+  // per 2^24-point batch: 2^17 / 2^18 points 176 / 207 -> 167 / 170 us; c2r within 1-2%); for
+  // complex rows of whole lines it measured neutral to 4% slower (2^15..2^18 points)
   StageArgs b = a;
   b.xcd = 0;
-  const int64_t nx = (a.nsub + a.G - 1) / a.G;
+  int64_t nx = (a.nsub + a.G - 1) / a.G;
+  if ((a.io_in != 0 || a.io_out == 2) && a.nsub >= 8 * (int64_t)a.G) {
+    b.xcd = 1;
+    nx = (nx + 7) / 8 * 8;
+  }
   const size_t lds = (size_t)a.G * reg_lds_doubles(a.L) * sizeof(double);
   const int bt = a.L / 8 * a.G;
   void *args[] = {(void *)&in, (void *)&out, (void *)&tw, (void *)&b};
