@@ -109,7 +109,8 @@ int xcd_remap(const StageArgs &a, int G) {
     const char *e = std::getenv("MFFT_XCD");
     return e ? std::atoi(e) : 1;
   }();
-  const bool split_lines = a.gfast_in && a.in_e % 8 != 0;
+  // (a 128-byte line holds 8 complex values, or 16 doubles of real input)
+  const bool split_lines = a.gfast_in && a.in_e % (a.io_in == 1 ? 16 : 8) != 0;
   return on && (a.kb_u != 0 || split_lines) && a.nsub >= 8 * (int64_t)G ? 1 : 0;
 }
 
