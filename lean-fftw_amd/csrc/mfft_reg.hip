@@ -543,7 +543,9 @@ __global__ __launch_bounds__(L / E * G, 4) void stage_reg_kernel(const double2 *
 // whether it is the one the planner prefers for that length. Measured on MI355X (256 MB
 // batches; per stage): 128 points 112 / 102 / 104 us at G = 8 / 16 / 32, 256 points
 // 108 / 103 / 118 us at 8 / 16 / 32, 512 points 108 / 120 us at 8 / 16 (the LDS-tile stage
-// kernel: 107, 113, 116 us). Wider rows lose more to occupancy than they gain.
+// kernel: 107, 113, 116 us), 1024 points at G = 4 with the XCD remap (three 512-thread
+// workgroups per CU against one of 1024 threads at G = 8; 2-D 1024 x 1024, columns then rows:
+// 229-235 -> 200 us per 2^24 elements). Wider rows lose more to occupancy than they gain.
 struct StageSchedule {
   int L, G, npass;
   int rs[4];
@@ -563,8 +565,8 @@ const StageSchedule kStages[] = {
     {256, 32, 3, {8, 8, 4}, false, MFFT_SK(256, 32, 8, 8, 4)},
     {512, 8, 3, {8, 8, 8}, true, MFFT_SK(512, 8, 8, 8, 8)},
     {512, 16, 3, {8, 8, 8}, false, MFFT_SK(512, 16, 8, 8, 8)},
-    {1024, 4, 4, {8, 8, 8, 2}, false, MFFT_SK(1024, 4, 8, 8, 8, 2)},
-    {1024, 8, 4, {8, 8, 8, 2}, true, MFFT_SK(1024, 8, 8, 8, 8, 2)},
+    {1024, 4, 4, {8, 8, 8, 2}, true, MFFT_SK(1024, 4, 8, 8, 8, 2)},
+    {1024, 8, 4, {8, 8, 8, 2}, false, MFFT_SK(1024, 8, 8, 8, 8, 2)},
     {2048, 4, 4, {8, 8, 8, 4}, true, MFFT_SK(2048, 4, 8, 8, 8, 4)},
 };
 #undef MFFT_SK
@@ -599,15 +601,18 @@ hipError_t launch_reg_stage(bool fwd, const StageArgs &a, int64_t ntrans, const 
     if (done.insert(k).second)
       (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytesMax);
   }
-  // XCD remap (neighbouring tiles of a row on one XCD) only for the real-data maps, whose rows
-  // of 8 doubles (real input) or of n/2 + 1 values per transform (the half output) leave
-  // 128-byte lines shared between neighbouring tiles, which then meet in one L2 (MI355X, r2c
-  // per 2^24-point batch: 2^17 / 2^18 points 176 / 207 -> 167 / 170 us; c2r within 1-2%); for
-  // complex rows of whole lines it measured neutral to 4% slower (2^15..2^18 points)
+  // XCD remap (neighbouring tiles of a row on one XCD) where a tile's rows leave 128-byte
+  // lines shared with the neighbouring tile, which then meet in one L2: the real-data maps
+  // (rows of 8 doubles of real input, or n/2 + 1 values per transform of half output; MI355X,
+  // r2c per 2^24-point batch: 2^17 / 2^18 points 176 / 207 -> 167 / 170 us, c2r within 1-2%)
+  // and complex rows of fewer than 8 values (the 1024-point stage at G = 4: 2-D 1024 x 1024
+  // 229-235 -> 200 us). For complex rows of whole lines it measured neutral to 4% slower
+  // (2^15..2^18 points).
   StageArgs b = a;
   b.xcd = 0;
   int64_t nx = (a.nsub + a.G - 1) / a.G;
-  if ((a.io_in != 0 || a.io_out == 2) && a.nsub >= 8 * (int64_t)a.G) {
+  const bool part_rows = (a.gfast_in || a.gfast_out) && a.G < 8;
+  if ((a.io_in != 0 || a.io_out == 2 || part_rows) && a.nsub >= 8 * (int64_t)a.G) {
     b.xcd = 1;
     nx = (nx + 7) / 8 * 8;
   }
