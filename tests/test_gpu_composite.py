@@ -103,6 +103,30 @@ def test_real_round_trip_full_size(lf):
     assert err < 1e-15 * 18
 
 
+@pytest.mark.parametrize("n", [2100, 4095, 10368, 75600, 165375])
+def test_real_full_batch_fused_maps(lf, oracle, n):
+    """The fused real-data paths of sizes that are not powers of two (staged tiles with the maps
+    in their copies: 2100, 4095; tile stages with the maps in their first / last passes: 10368,
+    75600, 165375) at the full batch of 2^24 / n transforms: c2r(r2c(x)) = n x, and the first,
+    middle and last transforms of each direction bit for bit against the oracle."""
+    hm = (1 << 24) // n
+    g = torch.Generator(DEV).manual_seed(n)
+    x = torch.rand(hm, n, dtype=torch.float64, device=DEV, generator=g) - 0.5
+    fwd, bwd = lf.RealPlan(n, lf.FORWARD, hm), lf.RealPlan(n, lf.BACKWARD, hm)
+    h = fwd.exec(x)
+    y = bwd.exec(h)
+    torch.cuda.synchronize()
+    err = float(torch.linalg.vector_norm(y / n - x) / torch.linalg.vector_norm(x))
+    assert err < 1e-15 * 20, err
+    rows = [0, hm // 2, hm - 1]
+    xs = x[rows].cpu().numpy()
+    full = oracle.ref_transform(True, fwd.radices, xs.astype(np.complex128))
+    assert bits_equal(h[rows].cpu().numpy(), full[:, : n // 2 + 1]), (n, "r2c")
+    hs = h[rows].cpu().numpy()
+    back = oracle.ref_transform(False, bwd.radices, herm_ext(hs, n))
+    assert bits_equal(y[rows].cpu().numpy(), np.ascontiguousarray(back.real)), (n, "c2r")
+
+
 def test_real_plan_rejects_in_place_and_composite_only_entry_points(lf):
     plan = lf.RealPlan(64, lf.FORWARD, 2)
     assert lf.lib().mfft_plan_kind(plan._h) == 1
