@@ -604,7 +604,9 @@ hipError_t launch_reg_stage(bool fwd, const StageArgs &a, int64_t ntrans, const 
   // XCD remap (neighbouring tiles of a row on one XCD) where a tile's rows leave 128-byte
   // lines shared with the neighbouring tile, which then meet in one L2: the real-data maps
   // (rows of 8 doubles of real input, or n/2 + 1 values per transform of half output; MI355X,
-  // r2c per 2^24-point batch: 2^17 / 2^18 points 176 / 207 -> 167 / 170 us, c2r within 1-2%)
+  // r2c per 2^24-point batch: 2^17 / 2^18 points 176 / 207 -> 167 / 170 us, c2r within 1-2%;
+  // c2r's real-part rows of 8 doubles measured within run-to-run noise with it, so they keep
+  // the plain order)
   // and complex rows of fewer than 8 values (the 1024-point stage at G = 4: 2-D 1024 x 1024
   // 229-235 -> 200 us). For complex rows of whole lines it measured neutral to 4% slower
   // (2^15..2^18 points).
