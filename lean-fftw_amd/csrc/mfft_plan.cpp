@@ -1512,16 +1512,15 @@ namespace {
 // most balanced split into two, or (no two-way split fits the LDS) of three.
 bool estimate_schedule(const mfft_plan *p, std::vector<int> &rs, int &split, int &split2) {
   const int64_t n = p->n;
-  const bool pow2 = (n & (n - 1)) == 0;
   const bool contig = p->istride == 1 && p->ostride == 1;
   split = split2 = 0;
   // columns too long for a tile of 4 (64-byte rows) run as two stages of wider rows rather
   // than one stage of narrower rows (MI355X, 2-D columns: 4096 points two stages 274-304 us
-  // against 868), and so do power-of-two columns too long for a tile of 8, whose stages have
-  // column tiles (2048 x 2048: 285-287 us as 32 x 64 against 316-322 in one stage of 4-wide
-  // rows)
-  const bool col_two =
-      columns_layout(p) && (4 * n > mfft::BIG_ELEMS || (pow2 && 8 * n > mfft::BIG_ELEMS));
+  // against 868). 2048-point columns take one register stage of 4-wide rows, whose tiles the
+  // XCD remap keeps beside the neighbours they share lines with, and which carries the batch
+  // of arrays in one launch (2-D 2048 x 2048 x 4: 295 us as two column stages of 32 x 64 per
+  // array, 252-253 in one; before the remap one stage measured 316-322 against 285-287)
+  const bool col_two = columns_layout(p) && 4 * n > mfft::BIG_ELEMS;
   if (single_stage(n, p->allow_big, contig) && !col_two) {
     rs = p->allow_big && contig && reg_size(n) ? reg_radices(n, p->fuse)
                                                : choose_radices(n, !p->allow_big);

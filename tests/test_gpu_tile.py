@@ -85,11 +85,13 @@ def run_case(lf, oracle, N, hm, fwd, rs, split=None):
     assert bits_equal(xd.cpu().numpy(), ref), (N, plan.radices, plan.stages, "in place")
 
 
-# columns of a row-major (N, hm) array: single-stage tiles of 32 adjacent transforms of 64
-# points (rows 40 elements apart, and 300 apart: not whole 128-byte lines, so the launch is
-# XCD-remapped over a grid padded from 10 to 16 tiles), and both stages of the two-stage
-# column plans of 4096 (64 x 64) and 2048 (32 x 64) points
-COLUMN_TILE_CASES = [(64, 40), (64, 300), (4096, 40), (2048, 40)]
+# columns of a row-major (N, hm) array: single-stage tiles of 32 adjacent transforms of 64 and
+# of 32 points (rows 40 elements apart, and 300 apart: not whole 128-byte lines, so the launch
+# is XCD-remapped over a grid padded from 10 to 16 tiles), both stages of the two-stage column
+# plan of 4096 (64 x 64) points, and the single register stage of 2048-point columns (tiles of
+# 4 columns, XCD-remapped)
+COLUMN_TILE_CASES = [(64, 40), (64, 300), (32, 300), (4096, 40), (2048, 40)]
+COLUMN_STAGES = {4096: [2, 2], 2048: [4]}
 
 
 @pytest.mark.parametrize("N,hm", COLUMN_TILE_CASES)
@@ -99,7 +101,7 @@ def test_column_tile_kernels_bit_exact(lf, oracle, N, hm, fwd):
     xt = torch.from_numpy(np.ascontiguousarray(x.T)).to(DEV)
     plan = lf.Plan(N, lf.FORWARD if fwd else lf.BACKWARD, hm, istride=hm, idist=1, ostride=hm,
                    odist=1)
-    assert N == 64 or plan.stages == [2, 2], (N, plan.radices, plan.stages)
+    assert plan.stages == COLUMN_STAGES.get(N, plan.stages), (N, plan.radices, plan.stages)
     ref = np.ascontiguousarray(oracle.ref_transform(fwd, plan.radices, x).T)
     out = torch.empty_like(xt)
     plan.call(xt, out)
